@@ -1,0 +1,200 @@
+"""Benchmark: cell-steps/s of the per-timestep hot path at 256^3 (config 2),
+the PCG's achieved HBM bandwidth, and the FP32-refined solver against the
+FP64 one at the same 1e-8 relative residual.
+
+A step is one Solver::step (RK3 semi-Lagrangian advection of q, u, v, w,
+inputs, divergence, Jacobi-PCG to 1e-8, projection) over the whole 256^3
+grid, state resident in HBM.  Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--n N] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "cell-steps/s at 256^3 + PCG HBM GB/s; FP32-refined vs FP64 at 1e-8 resid"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per owned cell per launch of each PCG kernel (FP64
+# words x 8; FP32 words x 4): spmv+dot reads p, writes q; update reads x, p,
+# r, q and writes x, r; direction reads r, p and writes p.
+WORDS = {"spmv": 2, "update": 6, "dir": 3}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=20)
+    return ap.parse_args()
+
+
+def make_solver(cfg):
+    from cajitafluids_amd import configs as K
+    from cajitafluids_amd.solver import Solver
+    s = Solver(cfg)
+    s.init_vortex_tubes(K.CONFIG2_SEED, 1.0)
+    return s
+
+
+def timed_run(sol, steps, warmup, timing=True):
+    from cajitafluids_amd import _lib
+    sol.setup()
+    for _ in range(warmup):
+        sol.step()
+    sol.sync()
+    sol.reset_kernel_time()
+    sol.set_kernel_timing(timing)
+    iters, outer, res = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = sol.step()
+        iters.append(st["iterations"])
+        outer.append(st["outer_iterations"])
+        res.append(st["final_residual"])
+    sol.sync()
+    t = time.perf_counter() - t0
+    kt = {k: sol.kernel_time(k) for k in range(8)}
+    sol.set_kernel_timing(False)
+    return t, iters, outer, res, kt
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; the "
+                          "multi-GPU design is described in DESIGN.md",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    from cajitafluids_amd import _lib, configs as K
+
+    n = args.n
+    ncell = n ** 3
+    cfg = K.vortex_tank_3d(n)
+    sol = make_solver(cfg)
+    t64, it64, _, res64, kt = timed_run(sol, args.steps, args.warmup)
+    dt_eff = sol.dt
+    sol.close()
+    del sol
+
+    value = ncell * args.steps / t64
+    names = {_lib.TK_SPMV: "spmv", _lib.TK_UPDATE: "update", _lib.TK_DIR: "dir"}
+    per = {}
+    for k, nm in names.items():
+        ms, cnt = kt[k]
+        if cnt:
+            per[nm] = dict(avg_us=1e3 * ms / cnt, launches=cnt, total_ms=ms,
+                           bytes=WORDS[nm] * 8 * ncell)
+    t_pcg = sum(p["total_ms"] for p in per.values()) / 1e3
+    total_iters = sum(it64)
+    pcg_gbs = 88.0 * ncell * total_iters / t_pcg / 1e9 if t_pcg > 0 else None
+    dom = max(per, key=lambda k: per[k]["total_ms"])
+    d = per[dom]
+    achieved = d["bytes"] / (d["avg_us"] * 1e-6) / 1e9
+    roofline = {"bound": "hbm", "kernel": f"k_pcg_{dom}", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None,
+                "bytes_per_launch": d["bytes"], "avg_us": round(d["avg_us"], 2)}
+    adv_ms, adv_n = kt[_lib.TK_ADVECT]
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t64 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded divergence-free vortex tubes (seed 1), sphere inflow",
+        "config": {"workload": f"config 2: 3-D {n}^3 MAC grid, FP64, cubic B-spline fields, "
+                               f"RK3, Jacobi-PCG to 1e-8, dt clamped to {dt_eff:.6g}",
+                   "grid": [n, n, n], "halo": cfg["halo"], "cg_tol": cfg["cg_tol"]},
+        "pcg": {"iterations_per_step": round(total_iters / args.steps, 1),
+                "iterations": it64, "gbs_11_words": round(pcg_gbs, 1) if pcg_gbs else None,
+                "frac_11_words": round(pcg_gbs / HBM_PEAK_GBS, 4) if pcg_gbs else None,
+                "time_share": round(t_pcg / t64, 4),
+                "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv)
+                                for kk, vv in v.items()} for k, v in per.items()},
+                "max_final_residual": max(res64)},
+        "advect": {"avg_ms": round(adv_ms / adv_n, 3) if adv_n else None},
+        "roofline": roofline,
+    }
+
+    if not args.no_fp32:
+        cfg32 = K.vortex_tank_3d(n, precision=K.FP32_REFINED)
+        s32 = make_solver(cfg32)
+        t32, it32, outer32, res32, kt32 = timed_run(s32, args.steps, args.warmup)
+        s32.close()
+        del s32
+        v32 = ncell * args.steps / t32
+        out["fp32_refined"] = {
+            "value": v32, "unit": "cell-steps/s", "speedup_vs_fp64": round(v32 / value, 3),
+            "ms_per_step": 1e3 * t32 / args.steps,
+            "inner_iterations_per_step": round(sum(it32) / args.steps, 1),
+            "outer_passes_per_step": round(sum(outer32) / args.steps, 2),
+            "max_final_residual": max(res32),
+            "inner_rtol": cfg32["inner_tol"],
+        }
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cfg, n, total_iters / args.steps, args.cpu_iters)
+
+    print(json.dumps(out))
+
+
+def cpu_baseline(cfg, n, iters_per_step, cpu_iters):
+    """The oracle (a plain-C restatement with the reference's stored-stencil
+    traffic) timed on the host: one advection + inputs + RHS + projection,
+    and cpu_iters PCG iterations, on the same 256^3 state; the step time is
+    extrapolated with the GPU's PCG iteration count (the two paths take the
+    same iterations)."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_lib as O
+    from cajitafluids_amd import configs as K
+    from cajitafluids_amd.solver import Solver
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    sol = Solver(cfg)
+    sol.init_vortex_tubes(K.CONFIG2_SEED, 1.0)
+    sol.setup()
+    c2 = dict(cfg)
+    c2["cg_max_iter"] = cpu_iters
+    orc = O.Oracle(c2)
+    for e in range(4):
+        orc.field(e)[...] = sol.get(e)
+    sol.close()
+    t0 = time.perf_counter()
+    orc.time_integrate()
+    orc.add_inputs()
+    orc.build_rhs()
+    t1 = time.perf_counter()
+    orc.pressure_solve(O.FP64)   # stops at cpu_iters (non-converged)
+    t2 = time.perf_counter()
+    orc.apply_pressure()
+    t3 = time.perf_counter()
+    t_iter = (t2 - t1) / cpu_iters
+    t_step = (t1 - t0) + (t3 - t2) + iters_per_step * t_iter
+    orc.close()
+    return {"value": n ** 3 / t_step, "unit": "cell-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"one {n}^3 step's advection+inputs+RHS+projection and {cpu_iters} "
+                      f"PCG iterations of the oracle (OpenMP, {threads} threads), "
+                      f"step time extrapolated with {iters_per_step:.0f} iterations/step",
+            "sample_seconds": round(t3 - t0, 2), "pcg_iter_ms": round(1e3 * t_iter, 2)}
+
+
+if __name__ == "__main__":
+    main()

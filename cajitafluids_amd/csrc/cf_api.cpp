@@ -1,0 +1,412 @@
+// cf_api.cpp -- the C ABI (include/cf_api.h): a thin shell over
+// cfmi::Solver that turns exceptions into status codes.
+#include "cf_api.h"
+
+#include <cstring>
+#include <string>
+
+#include "cf_solver.hpp"
+
+using cfmi::CfError;
+
+struct cf_solver
+{
+    cfmi::Solver* s = nullptr;
+    std::string err;
+};
+
+static thread_local std::string g_create_error;
+
+template <class F>
+static cf_status guarded(cf_solver* h, F&& f)
+{
+    if (!h || !h->s)
+        return CF_EINVAL;
+    try {
+        return f(*h->s);
+    } catch (const CfError& e) {
+        h->err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        h->err = "host allocation failed";
+        return CF_ENOMEM;
+    } catch (const std::exception& e) {
+        h->err = e.what();
+        return CF_EDEVICE;
+    }
+}
+
+static bool valid_entity(const cfmi::Geo& g, int e)
+{
+    return e >= 0 && e <= g.dim;
+}
+
+extern "C" {
+
+cf_status cf_layout_query(const cf_config* cfg, int entity, int ghost_ext[3],
+                          int own_lo[3], int own_hi[3])
+{
+    if (!cfg)
+        return CF_EINVAL;
+    cfmi::Geo g;
+    const char* why;
+    cf_status rc = cfmi::build_geometry(*cfg, g, &why);
+    if (rc != CF_OK)
+        return rc;
+    if (!valid_entity(g, entity))
+        return CF_EINVAL;
+    for (int d = 0; d < 3; ++d) {
+        if (ghost_ext)
+            ghost_ext[d] = g.ext[entity][d];
+        if (own_lo)
+            own_lo[d] = g.own_lo(d);
+        if (own_hi)
+            own_hi[d] = g.own_hi[entity][d];
+    }
+    return CF_OK;
+}
+
+cf_status cf_geometry_query(const cf_config* cfg, double* dx, double* dt)
+{
+    if (!cfg)
+        return CF_EINVAL;
+    cfmi::Geo g;
+    const char* why;
+    cf_status rc = cfmi::build_geometry(*cfg, g, &why);
+    if (rc != CF_OK)
+        return rc;
+    if (dx)
+        *dx = g.dx;
+    if (dt)
+        *dt = g.dt;
+    return CF_OK;
+}
+
+cf_status cf_create(const cf_config* cfg, cf_solver** out)
+{
+    if (!out)
+        return CF_EINVAL;
+    *out = nullptr;
+    if (!cfg) {
+        g_create_error = "null config";
+        return CF_EINVAL;
+    }
+    cf_solver* h = new (std::nothrow) cf_solver();
+    if (!h)
+        return CF_ENOMEM;
+    try {
+        h->s = new cfmi::Solver(*cfg);
+    } catch (const CfError& e) {
+        g_create_error = e.what();
+        delete h;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_create_error = "host allocation failed";
+        delete h;
+        return CF_ENOMEM;
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        delete h;
+        return CF_EDEVICE;
+    }
+    *out = h;
+    return CF_OK;
+}
+
+void cf_destroy(cf_solver* h)
+{
+    if (!h)
+        return;
+    try {
+        delete h->s;
+    } catch (...) {
+    }
+    delete h;
+}
+
+const char* cf_last_error(const cf_solver* h)
+{
+    if (!h)
+        return g_create_error.c_str();
+    return h->err.c_str();
+}
+
+cf_status cf_setup(cf_solver* h, cf_solve_stats* st)
+{
+    return guarded(h, [&](cfmi::Solver& s) { return s.setup(st); });
+}
+
+cf_status cf_step(cf_solver* h, cf_solve_stats* st)
+{
+    return guarded(h, [&](cfmi::Solver& s) { return s.step(st); });
+}
+
+cf_status cf_run(cf_solver* h, int nsteps, cf_solve_stats* last)
+{
+    return guarded(h, [&](cfmi::Solver& s) {
+        if (nsteps < 0)
+            throw CfError(CF_EINVAL, "nsteps must be >= 0");
+        return s.run(nsteps, last);
+    });
+}
+
+double cf_time(const cf_solver* h) { return h && h->s ? h->s->time() : 0.0; }
+double cf_dt(const cf_solver* h) { return h && h->s ? h->s->geo().dt : 0.0; }
+double cf_dx(const cf_solver* h) { return h && h->s ? h->s->geo().dx : 0.0; }
+
+cf_status cf_time_integrate(cf_solver* h)
+{
+    return guarded(h, [&](cfmi::Solver& s) {
+        s.timeIntegrate();
+        return CF_OK;
+    });
+}
+
+cf_status cf_add_inputs(cf_solver* h)
+{
+    return guarded(h, [&](cfmi::Solver& s) {
+        s.addInputs();
+        return CF_OK;
+    });
+}
+
+cf_status cf_correct_velocity(cf_solver* h, cf_solve_stats* st)
+{
+    return guarded(h, [&](cfmi::Solver& s) { return s.vc().correctVelocity(st); });
+}
+
+cf_status cf_build_rhs(cf_solver* h)
+{
+    return guarded(h, [&](cfmi::Solver& s) {
+        s.vc().buildRHS();
+        return CF_OK;
+    });
+}
+
+cf_status cf_rhs_remove_mean(cf_solver* h)
+{
+    return guarded(h, [&](cfmi::Solver& s) {
+        s.vc().removeMean();
+        return CF_OK;
+    });
+}
+
+cf_status cf_pressure_solve(cf_solver* h, int precision, cf_solve_stats* st)
+{
+    return guarded(h, [&](cfmi::Solver& s) {
+        return s.vc().solve(precision, s.vc().rhs(), s.vc().lhs(), st);
+    });
+}
+
+cf_status cf_pressure_solve_ext(cf_solver* h, const double* rhs_dev,
+                                double* lhs_dev, int precision,
+                                cf_solve_stats* st)
+{
+    return guarded(h, [&](cfmi::Solver& s) {
+        if (!rhs_dev || !lhs_dev || rhs_dev == lhs_dev)
+            throw CfError(CF_EINVAL, "rhs and lhs must be distinct device arrays");
+        return s.vc().solve(precision, rhs_dev, lhs_dev, st);
+    });
+}
+
+cf_status cf_apply_pressure(cf_solver* h)
+{
+    return guarded(h, [&](cfmi::Solver& s) {
+        s.vc().applyPressure();
+        return CF_OK;
+    });
+}
+
+static void fill_view(const cfmi::Geo& g, double* data, int entity, cf_view* out)
+{
+    out->data = data;
+    out->offset = g.sx;
+    out->pitch_x = g.px;
+    out->pitch_y = g.py;
+    for (int d = 0; d < 3; ++d) {
+        out->ghost_ext[d] = g.ext[entity][d];
+        out->own_lo[d] = g.own_lo(d);
+        out->own_hi[d] = g.own_hi[entity][d];
+    }
+}
+
+cf_status cf_field_view(cf_solver* h, int entity, int version, cf_view* out)
+{
+    return guarded(h, [&](cfmi::Solver& s) {
+        if (!out || !valid_entity(s.geo(), entity) || (version != 0 && version != 1))
+            throw CfError(CF_EINVAL, "bad entity/version/out");
+        fill_view(s.geo(), s.pm().get(entity, version), entity, out);
+        return CF_OK;
+    });
+}
+
+cf_status cf_pressure_view(cf_solver* h, int which, cf_view* out)
+{
+    return guarded(h, [&](cfmi::Solver& s) {
+        if (!out || (which != 0 && which != 1))
+            throw CfError(CF_EINVAL, "which must be 0 (lhs) or 1 (rhs)");
+        fill_view(s.geo(), which == 0 ? s.vc().lhs() : s.vc().rhs(), CF_CELL, out);
+        return CF_OK;
+    });
+}
+
+// Dense ghosted host array <-> padded device layout, one 2-D copy per plane.
+static void copy_dense(cfmi::Solver& s, double* dev, int entity, void* host,
+                       bool to_device)
+{
+    const cfmi::Geo& g = s.geo();
+    const int E0 = g.ext[entity][0], E1 = g.ext[entity][1], E2 = g.ext[entity][2];
+    for (int k = 0; k < E2; ++k) {
+        double* d = dev + g.sx + g.px * g.py * (int64_t)k;
+        char* hp = (char*)host + sizeof(double) * (size_t)E0 * E1 * k;
+        if (to_device)
+            CF_HIP(hipMemcpy2DAsync(d, g.px * sizeof(double), hp,
+                                    E0 * sizeof(double), E0 * sizeof(double), E1,
+                                    hipMemcpyHostToDevice, s.stream()));
+        else
+            CF_HIP(hipMemcpy2DAsync(hp, E0 * sizeof(double), d,
+                                    g.px * sizeof(double), E0 * sizeof(double),
+                                    E1, hipMemcpyDeviceToHost, s.stream()));
+    }
+    CF_HIP(hipStreamSynchronize(s.stream()));
+}
+
+cf_status cf_copy_in(cf_solver* h, int entity, int version, const double* host)
+{
+    return guarded(h, [&](cfmi::Solver& s) {
+        if (!host || !valid_entity(s.geo(), entity) || (version != 0 && version != 1))
+            throw CfError(CF_EINVAL, "bad entity/version/host");
+        copy_dense(s, s.pm().get(entity, version), entity, (void*)host, true);
+        return CF_OK;
+    });
+}
+
+cf_status cf_copy_out(cf_solver* h, int entity, int version, double* host)
+{
+    return guarded(h, [&](cfmi::Solver& s) {
+        if (!host || !valid_entity(s.geo(), entity) || (version != 0 && version != 1))
+            throw CfError(CF_EINVAL, "bad entity/version/host");
+        copy_dense(s, s.pm().get(entity, version), entity, host, false);
+        return CF_OK;
+    });
+}
+
+cf_status cf_copy_pressure_in(cf_solver* h, int which, const double* host)
+{
+    return guarded(h, [&](cfmi::Solver& s) {
+        if (!host || (which != 0 && which != 1))
+            throw CfError(CF_EINVAL, "which must be 0 (lhs) or 1 (rhs)");
+        copy_dense(s, which == 0 ? s.vc().lhs() : s.vc().rhs(), CF_CELL,
+                   (void*)host, true);
+        return CF_OK;
+    });
+}
+
+cf_status cf_copy_pressure_out(cf_solver* h, int which, double* host)
+{
+    return guarded(h, [&](cfmi::Solver& s) {
+        if (!host || (which != 0 && which != 1))
+            throw CfError(CF_EINVAL, "which must be 0 (lhs) or 1 (rhs)");
+        copy_dense(s, which == 0 ? s.vc().lhs() : s.vc().rhs(), CF_CELL, host,
+                   false);
+        return CF_OK;
+    });
+}
+
+cf_status cf_advance(cf_solver* h, int entity)
+{
+    return guarded(h, [&](cfmi::Solver& s) {
+        if (!valid_entity(s.geo(), entity))
+            throw CfError(CF_EINVAL, "bad entity");
+        s.pm().advance(entity);
+        return CF_OK;
+    });
+}
+
+cf_status cf_fill_ghosts(cf_solver* h, int version)
+{
+    return guarded(h, [&](cfmi::Solver& s) {
+        if (version != 0 && version != 1)
+            throw CfError(CF_EINVAL, "bad version");
+        s.pm().gather(version);
+        return CF_OK;
+    });
+}
+
+cf_status cf_sync(cf_solver* h)
+{
+    return guarded(h, [&](cfmi::Solver& s) {
+        CF_HIP(hipStreamSynchronize(s.stream()));
+        return CF_OK;
+    });
+}
+
+cf_status cf_init_vortex_tubes(cf_solver* h, uint64_t seed, double peak)
+{
+    return guarded(h, [&](cfmi::Solver& s) {
+        s.initVortexTubes(seed, peak);
+        return CF_OK;
+    });
+}
+
+cf_status cf_init_blobs(cf_solver* h, uint64_t seed, int nblobs)
+{
+    return guarded(h, [&](cfmi::Solver& s) {
+        s.initBlobs(seed, nblobs);
+        return CF_OK;
+    });
+}
+
+cf_status cf_init_random_faces(cf_solver* h, uint64_t seed)
+{
+    return guarded(h, [&](cfmi::Solver& s) {
+        s.initRandomFaces(seed);
+        return CF_OK;
+    });
+}
+
+cf_status cf_set_kernel_timing(cf_solver* h, int enable)
+{
+    return guarded(h, [&](cfmi::Solver& s) {
+        s.timer().enabled = enable != 0;
+        return CF_OK;
+    });
+}
+
+cf_status cf_kernel_time(cf_solver* h, int kernel, double* total_ms,
+                         int64_t* launches)
+{
+    return guarded(h, [&](cfmi::Solver& s) {
+        if (kernel < 0 || kernel >= cfmi::TK_COUNT)
+            throw CfError(CF_EINVAL, "bad kernel id");
+        CF_HIP(hipStreamSynchronize(s.stream()));
+        s.timer().resolve(INT32_MAX);
+        if (total_ms)
+            *total_ms = s.timer().total_ms[kernel];
+        if (launches)
+            *launches = s.timer().launches[kernel];
+        return CF_OK;
+    });
+}
+
+cf_status cf_reset_kernel_time(cf_solver* h)
+{
+    return guarded(h, [&](cfmi::Solver& s) {
+        s.timer().reset();
+        return CF_OK;
+    });
+}
+
+cf_status cf_adjoint_run(cf_solver* h, int nsteps, int ckpt,
+                         const double* dq_T_host, double* du0_host,
+                         double* dv0_host, double* dw0_host, double* dq0_host,
+                         double dvel[3])
+{
+    return guarded(h, [&](cfmi::Solver&) -> cf_status {
+        (void)nsteps; (void)ckpt; (void)dq_T_host; (void)du0_host;
+        (void)dv0_host; (void)dw0_host; (void)dq0_host; (void)dvel;
+        throw CfError(CF_ENOTIMPL, "adjoint not built yet");
+    });
+}
+
+} // extern "C"

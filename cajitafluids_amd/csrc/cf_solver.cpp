@@ -1,0 +1,519 @@
+// cf_solver.cpp -- host-side classes (Mesh, ProblemManager,
+// VelocityCorrector, TimeIntegrator, Solver) driving the HIP kernels.
+#include "cf_solver.hpp"
+
+#include <climits>
+#include <cstring>
+
+#include "cf_init.hpp"
+
+namespace cfmi
+{
+
+void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess)
+        throw CfError(e == hipErrorOutOfMemory ? CF_ENOMEM : CF_EDEVICE,
+                      std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+void DevArray<T>::alloc(int64_t count, hipStream_t s)
+{
+    release();
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, (size_t)count * sizeof(T));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw CfError(CF_ENOMEM, "hipMalloc of " +
+                                     std::to_string((size_t)count * sizeof(T)) +
+                                     " bytes failed");
+    }
+    ptr = (T*)p;
+    n = count;
+    CF_HIP(hipMemsetAsync(ptr, 0, (size_t)count * sizeof(T), s));
+}
+
+template struct DevArray<double>;
+template struct DevArray<float>;
+template struct DevArray<unsigned>;
+template struct DevArray<unsigned long long>;
+
+// ---- KernelTimer -----------------------------------------------------------
+KernelTimer::~KernelTimer()
+{
+    for (auto& p : pending_) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (auto e : pool_)
+        (void)hipEventDestroy(e);
+}
+
+hipEvent_t KernelTimer::get()
+{
+    if (!pool_.empty()) {
+        hipEvent_t e = pool_.back();
+        pool_.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    CF_HIP(hipEventCreate(&e));
+    return e;
+}
+
+int KernelTimer::begin(hipStream_t s)
+{
+    if (!enabled)
+        return -1;
+    Pending p;
+    p.a = get();
+    p.b = nullptr;
+    p.kernel = -1;
+    p.iter = 0;
+    CF_HIP(hipEventRecord(p.a, s));
+    pending_.push_back(p);
+    return (int)pending_.size() - 1;
+}
+
+void KernelTimer::end(int slot, hipStream_t s, int kernel, int iter)
+{
+    if (slot < 0)
+        return;
+    Pending& p = pending_[slot];
+    p.b = get();
+    p.kernel = kernel;
+    p.iter = iter;
+    CF_HIP(hipEventRecord(p.b, s));
+}
+
+void KernelTimer::resolve(int limit)
+{
+    for (auto& p : pending_) {
+        if (p.b && p.iter < limit) {
+            float ms = 0.f;
+            CF_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+            total_ms[p.kernel] += ms;
+            launches[p.kernel] += 1;
+        }
+        pool_.push_back(p.a);
+        if (p.b)
+            pool_.push_back(p.b);
+    }
+    pending_.clear();
+}
+
+void KernelTimer::reset()
+{
+    for (int k = 0; k < TK_COUNT; ++k) {
+        total_ms[k] = 0.0;
+        launches[k] = 0;
+    }
+}
+
+// ---- Mesh --------------------------------------------------------------------
+Mesh::Mesh(const cf_config& c)
+{
+    const char* why = "";
+    cf_status rc = build_geometry(c, _geo, &why);
+    if (rc != CF_OK)
+        throw CfError(rc, why);
+}
+
+// ---- ProblemManager ------------------------------------------------------------
+ProblemManager::ProblemManager(const Mesh& mesh, hipStream_t s)
+    : _mesh(mesh), _s(s)
+{
+    const Geo& g = mesh.geo();
+    // ProblemManager.hpp:136-165: every array zeroed over its ghosted extent.
+    for (int e = 0; e <= g.dim; ++e)
+        for (int v = 0; v < 2; ++v) {
+            _f[e][v].reset(new DevArray<double>());
+            _f[e][v]->alloc(g.total, s);
+        }
+}
+
+Fields ProblemManager::fields(int version) const
+{
+    Fields f;
+    const int D = _mesh.geo().dim;
+    for (int e = 0; e < 4; ++e)
+        f.e[e] = e <= D ? get(e, version) : nullptr;
+    return f;
+}
+
+void ProblemManager::gather(int version)
+{
+    // ProblemManager.hpp:394-403 on one rank: periodic axes copy, walls keep
+    // their zero ghosts.
+    const Geo& g = _mesh.geo();
+    if (!(g.periodic[0] || g.periodic[1] || g.periodic[2]))
+        return;
+    for (int e = 0; e <= g.dim; ++e)
+        launch_ghost_fill(g, e, get(e, version), _s);
+}
+
+// ---- TimeIntegrator ------------------------------------------------------------
+namespace TimeIntegrator
+{
+void step(ProblemManager& pm, hipStream_t s, KernelTimer& timer)
+{
+    const Geo& g = pm.mesh().geo();
+    pm.gather(CF_CURRENT);
+    int slot = timer.begin(s);
+    launch_advect(g, pm.fields(CF_CURRENT), pm.fields(CF_NEXT), s);
+    timer.end(slot, s, TK_ADVECT, 0);
+    for (int e = 0; e <= g.dim; ++e)
+        pm.advance(e);
+}
+} // namespace TimeIntegrator
+
+// ---- VelocityCorrector -----------------------------------------------------------
+VelocityCorrector::VelocityCorrector(ProblemManager& pm, const cf_config& cfg,
+                                     hipStream_t s, KernelTimer& timer)
+    : _pm(pm), _cfg(cfg), _s(s), _timer(timer), _g(pm.mesh().geo())
+{
+    const int64_t n = _g.total;
+    // VelocityCorrector.hpp:83-90: lhs / rhs zeroed over the ghosted extent.
+    _lhs.alloc(n, s);
+    _rhs.alloc(n, s);
+    _r.alloc(n, s);
+    _p.alloc(n, s);
+    _q.alloc(n, s);
+    _red_partials.alloc((int64_t)kRedBlocks * 4 * 2, s);
+    _red_counter.alloc(1, s);
+    _scratch.alloc(8, s);
+    void* p = nullptr;
+    CF_HIP(hipMalloc(&p, sizeof(PcgState)));
+    _st = (PcgState*)p;
+    CF_HIP(hipMalloc(&p, sizeof(RefineState)));
+    _rs = (RefineState*)p;
+    CF_HIP(hipHostMalloc(&p, sizeof(PcgState), hipHostMallocDefault));
+    _st_host = (PcgState*)p;
+    CF_HIP(hipHostMalloc(&p, sizeof(RefineState), hipHostMallocDefault));
+    _rs_host = (RefineState*)p;
+    CF_HIP(hipMemsetAsync(_st, 0, sizeof(PcgState), s));
+    CF_HIP(hipMemsetAsync(_rs, 0, sizeof(RefineState), s));
+    if (cfg.precision == CF_FP32_REFINED)
+        ensure32();
+}
+
+VelocityCorrector::~VelocityCorrector()
+{
+    if (_st)
+        (void)hipFree(_st);
+    if (_rs)
+        (void)hipFree(_rs);
+    if (_st_host)
+        (void)hipHostFree(_st_host);
+    if (_rs_host)
+        (void)hipHostFree(_rs_host);
+}
+
+void VelocityCorrector::ensure32()
+{
+    if (_d32.ptr)
+        return;
+    const int64_t n = _g.total;
+    _d32.alloc(n, _s);
+    _r32.alloc(n, _s);
+    _p32.alloc(n, _s);
+    _q32.alloc(n, _s);
+}
+
+void VelocityCorrector::readState(PcgState& host)
+{
+    CF_HIP(hipMemcpyAsync(_st_host, _st, sizeof(PcgState),
+                          hipMemcpyDeviceToHost, _s));
+    CF_HIP(hipStreamSynchronize(_s));
+    host = *_st_host;
+}
+
+void VelocityCorrector::buildRHS()
+{
+    // VelocityCorrector.hpp:182-212.
+    _pm.gather(CF_CURRENT);
+    launch_divergence(_g, _pm.fields(CF_CURRENT), _rhs.ptr, _s);
+}
+
+void VelocityCorrector::removeMean()
+{
+    RedBuf rb{_red_partials.ptr, _red_counter.ptr};
+    launch_rhs_sum(_g, _rhs.ptr, _scratch.ptr, rb, _s);
+    launch_rhs_shift(_g, _rhs.ptr, _scratch.ptr, _s);
+}
+
+void VelocityCorrector::applyPressure()
+{
+    // VelocityCorrector.hpp:214-264: pressure halo, then the face updates.
+    if (_g.periodic[0] || _g.periodic[1] || _g.periodic[2])
+        launch_ghost_fill(_g, CF_CELL, _lhs.ptr, _s);
+    launch_project(_g, _pm.fields(CF_CURRENT), _lhs.ptr, _s);
+}
+
+// Launch batches of iterations without waiting; poll the device flag once
+// per batch.  The first batch is sized from the previous solve's count so
+// that few no-op launches follow convergence.
+int VelocityCorrector::runPcg64(const double* b, double* x, double tol,
+                                int maxit)
+{
+    RedBuf rb{_red_partials.ptr, _red_counter.ptr};
+    launch_pcg_init(_g, b, x, _r.ptr, _p.ptr, _st, tol, maxit, rb, _s);
+    PcgState hs;
+    if (maxit <= 0) {
+        readState(hs);
+        return hs.iter;
+    }
+    int launched = 0;
+    int batch = _predict64 > 4 ? _predict64 - 2 : 16;
+    for (;;) {
+        for (int c = 0; c < batch; ++c) {
+            const int it = launched + c + 1;
+            int t0 = _timer.begin(_s);
+            launch_pcg_spmv_dot(_g, _p.ptr, _q.ptr, _st, rb, _s);
+            _timer.end(t0, _s, TK_SPMV, it);
+            int t1 = _timer.begin(_s);
+            launch_pcg_update(_g, x, _r.ptr, _p.ptr, _q.ptr, _st, rb, _s);
+            _timer.end(t1, _s, TK_UPDATE, it);
+            int t2 = _timer.begin(_s);
+            launch_pcg_dir(_g, _r.ptr, _p.ptr, _st, _s);
+            // The direction update of the last iteration is a no-op.
+            _timer.end(t2, _s, TK_DIR, it + 1);
+        }
+        launched += batch;
+        readState(hs);
+        _timer.resolve(hs.done ? hs.iter + 1 : INT_MAX);
+        if (hs.done)
+            break;
+        batch = 4;
+    }
+    _predict64 = hs.iter;
+    return hs.iter;
+}
+
+cf_status VelocityCorrector::solve64(const double* b, double* x,
+                                     cf_solve_stats* st)
+{
+    const int its = runPcg64(b, x, _cfg.cg_tol, _cfg.cg_max_iter);
+    PcgState hs = *_st_host;
+    if (st) {
+        st->converged = hs.converged;
+        st->iterations = its;
+        st->outer_iterations = 0;
+        st->b_norm = hs.bnorm;
+        st->final_residual = hs.res;
+    }
+    return hs.converged ? CF_OK : CF_ENOCONV;
+}
+
+int VelocityCorrector::runPcg32(double tol, int maxit)
+{
+    RedBuf rb{_red_partials.ptr, _red_counter.ptr};
+    launch_pcg32_init(_g, _r32.ptr, _d32.ptr, _p32.ptr, _st, tol, maxit, rb, _s);
+    PcgState hs;
+    if (maxit <= 0) {
+        readState(hs);
+        return 0;
+    }
+    int launched = 0;
+    int batch = _predict32 > 4 ? _predict32 - 2 : 16;
+    for (;;) {
+        for (int c = 0; c < batch; ++c) {
+            const int it = launched + c + 1;
+            int t0 = _timer.begin(_s);
+            launch_pcg32_spmv_dot(_g, _p32.ptr, _q32.ptr, _st, rb, _s);
+            _timer.end(t0, _s, TK_SPMV32, it);
+            int t1 = _timer.begin(_s);
+            launch_pcg32_update(_g, _d32.ptr, _r32.ptr, _p32.ptr, _q32.ptr, _st,
+                                rb, _s);
+            _timer.end(t1, _s, TK_UPDATE32, it);
+            int t2 = _timer.begin(_s);
+            launch_pcg32_dir(_g, _r32.ptr, _p32.ptr, _st, _s);
+            _timer.end(t2, _s, TK_DIR32, it + 1);
+        }
+        launched += batch;
+        readState(hs);
+        _timer.resolve(hs.done ? hs.iter + 1 : INT_MAX);
+        if (hs.done)
+            break;
+        batch = 4;
+    }
+    _predict32 = hs.iter;
+    return hs.iter;
+}
+
+cf_status VelocityCorrector::solveRefined(const double* b, double* x,
+                                          cf_solve_stats* st)
+{
+    ensure32();
+    RedBuf rb{_red_partials.ptr, _red_counter.ptr};
+    launch_refine_init(_g, b, x, _r32.ptr, _rs, rb, _s);
+    auto read_rs = [&]() {
+        CF_HIP(hipMemcpyAsync(_rs_host, _rs, sizeof(RefineState),
+                              hipMemcpyDeviceToHost, _s));
+        CF_HIP(hipStreamSynchronize(_s));
+    };
+    read_rs();
+    cf_solve_stats loc;
+    std::memset(&loc, 0, sizeof(loc));
+    loc.b_norm = _rs_host->bnorm;
+    cf_status rc = CF_OK;
+    if (_rs_host->bnorm == 0.0) {
+        loc.converged = 1;
+        loc.final_residual = 0.0;
+    } else {
+        const int inner_max =
+            _cfg.inner_max_iter > 0 ? _cfg.inner_max_iter : _cfg.cg_max_iter;
+        for (;;) {
+            loc.final_residual = _rs_host->res;
+            if (_rs_host->res < _cfg.cg_tol) {
+                loc.converged = 1;
+                break;
+            }
+            if (loc.outer_iterations >= _cfg.max_outer ||
+                loc.iterations >= _cfg.cg_max_iter) {
+                rc = CF_ENOCONV;
+                break;
+            }
+            loc.iterations += runPcg32(_cfg.inner_tol, inner_max);
+            launch_refine_add(_g, x, _d32.ptr, _s);
+            int t = _timer.begin(_s);
+            launch_refine_residual(_g, b, x, _r32.ptr, _rs, rb, _s);
+            _timer.end(t, _s, TK_RESIDUAL, 0);
+            read_rs();
+            _timer.resolve(INT_MAX);
+            loc.outer_iterations += 1;
+        }
+    }
+    if (st)
+        *st = loc;
+    return rc;
+}
+
+cf_status VelocityCorrector::solve(int precision, const double* b, double* x,
+                                   cf_solve_stats* st)
+{
+    if (precision == CF_FP32_REFINED)
+        return solveRefined(b, x, st);
+    if (precision != CF_FP64)
+        throw CfError(CF_EINVAL, "precision must be CF_FP64 or CF_FP32_REFINED");
+    return solve64(b, x, st);
+}
+
+cf_status VelocityCorrector::correctVelocity(cf_solve_stats* st)
+{
+    // VelocityCorrector.hpp:266-282; a solve that does not converge aborts
+    // the correction before the projection (the reference's solver throws).
+    buildRHS();
+    cf_status rc = solve(_cfg.precision, _rhs.ptr, _lhs.ptr, st);
+    if (rc != CF_OK)
+        return rc;
+    applyPressure();
+    return CF_OK;
+}
+
+// ---- Solver ----------------------------------------------------------------------
+Solver::Solver(const cf_config& cfg) : _cfg(cfg)
+{
+    _mesh.reset(new Mesh(cfg));
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        throw CfError(CF_EDEVICE, "no HIP device available");
+    }
+    int dev = 0;
+    CF_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    CF_HIP(hipGetDeviceProperties(&prop, dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        throw CfError(CF_EDEVICE, std::string("device is ") + prop.gcnArchName +
+                                      "; this library is built for gfx950");
+    CF_HIP(hipStreamCreateWithFlags(&_s, hipStreamNonBlocking));
+    _pm.reset(new ProblemManager(*_mesh, _s));
+    _vc.reset(new VelocityCorrector(*_pm, _cfg, _s, _timer));
+    _vmax.alloc(1, _s);
+    CF_HIP(hipStreamSynchronize(_s));
+}
+
+Solver::~Solver()
+{
+    if (_s)
+        (void)hipStreamSynchronize(_s);
+    _vc.reset();
+    _pm.reset();
+    _vmax.release();
+    if (_s)
+        (void)hipStreamDestroy(_s);
+}
+
+void Solver::timeIntegrate() { TimeIntegrator::step(*_pm, _s, _timer); }
+
+void Solver::addInputs()
+{
+    launch_inputs(_mesh->geo(), _pm->fields(CF_CURRENT), _s);
+}
+
+cf_status Solver::setup(cf_solve_stats* st)
+{
+    // Solver.hpp:125-133.
+    addInputs();
+    return _vc->correctVelocity(st);
+}
+
+cf_status Solver::step(cf_solve_stats* st)
+{
+    // Solver.hpp:135-147.
+    timeIntegrate();
+    addInputs();
+    cf_status rc = _vc->correctVelocity(st);
+    _time += _mesh->geo().dt;
+    return rc;
+}
+
+cf_status Solver::run(int nsteps, cf_solve_stats* last)
+{
+    for (int n = 0; n < nsteps; ++n) {
+        cf_status rc = step(last);
+        if (rc != CF_OK)
+            return rc;
+    }
+    return CF_OK;
+}
+
+void Solver::initVortexTubes(uint64_t seed, double peak)
+{
+    const Geo& g = _mesh->geo();
+    InitParams P;
+    std::memset(&P, 0, sizeof(P));
+    for (int d = 0; d < 3; ++d) {
+        P.periodic[d] = g.periodic[d];
+        P.box_lo[d] = _cfg.lo[d];
+        P.box_hi[d] = _cfg.hi[d];
+    }
+    make_tube_params(g, _cfg, seed, P);
+    launch_init_tubes(g, P, _pm->fields(CF_CURRENT), _vmax.ptr, peak, _s);
+}
+
+void Solver::initBlobs(uint64_t seed, int nblobs)
+{
+    const Geo& g = _mesh->geo();
+    if (nblobs < 0 || nblobs > kMaxBlobs)
+        throw CfError(CF_EINVAL, "nblobs must be in [0, 32]");
+    InitParams P;
+    std::memset(&P, 0, sizeof(P));
+    for (int d = 0; d < 3; ++d) {
+        P.periodic[d] = g.periodic[d];
+        P.box_lo[d] = _cfg.lo[d];
+        P.box_hi[d] = _cfg.hi[d];
+    }
+    make_blob_params(g, _cfg, seed, nblobs, P);
+    launch_init_blobs(g, P, _pm->get(CF_CELL, CF_CURRENT), _s);
+}
+
+void Solver::initRandomFaces(uint64_t seed)
+{
+    launch_init_random(_mesh->geo(), seed, _pm->fields(CF_CURRENT), _s);
+}
+
+} // namespace cfmi

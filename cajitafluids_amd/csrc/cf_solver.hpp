@@ -1,0 +1,218 @@
+// cf_solver.hpp -- host-side classes mirroring the reference's interfaces:
+// Mesh (src/Mesh.hpp), ProblemManager (src/ProblemManager.hpp),
+// VelocityCorrector (src/VelocityCorrector.hpp), TimeIntegrator
+// (src/TimeIntegrator.hpp) and Solver (src/Solver.hpp).  The C ABI in
+// cf_api.cpp is a thin shell over Solver.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cf_api.h"
+#include "cf_geometry.hpp"
+#include "cf_kernels.hpp"
+
+namespace cfmi
+{
+
+struct CfError : std::runtime_error
+{
+    cf_status code;
+    CfError(cf_status c, const std::string& m) : std::runtime_error(m), code(c)
+    {
+    }
+};
+
+void hip_check(hipError_t e, const char* what);
+#define CF_HIP(call) ::cfmi::hip_check((call), #call)
+
+// One device allocation in the handle's padded layout.
+template <class T>
+struct DevArray
+{
+    T* ptr = nullptr;
+    int64_t n = 0;
+    DevArray() = default;
+    DevArray(const DevArray&) = delete;
+    DevArray& operator=(const DevArray&) = delete;
+    ~DevArray() { release(); }
+    void alloc(int64_t count, hipStream_t s);
+    void release()
+    {
+        if (ptr)
+            (void)hipFree(ptr);
+        ptr = nullptr;
+        n = 0;
+    }
+};
+
+// Per-kernel device time from HIP events on the handle's stream.
+enum TimedKernel
+{
+    TK_SPMV = 0,
+    TK_UPDATE = 1,
+    TK_DIR = 2,
+    TK_SPMV32 = 3,
+    TK_UPDATE32 = 4,
+    TK_DIR32 = 5,
+    TK_ADVECT = 6,
+    TK_RESIDUAL = 7,
+    TK_COUNT = 8
+};
+
+class KernelTimer
+{
+  public:
+    ~KernelTimer();
+    bool enabled = false;
+    // Record a start event; returns a slot for end().
+    int begin(hipStream_t s);
+    void end(int slot, hipStream_t s, int kernel, int iter);
+    // Resolve pending pairs (stream must be synchronised); pairs whose tag is
+    // not below `limit` were no-op launches after convergence and are dropped.
+    void resolve(int limit);
+    double total_ms[TK_COUNT] = {};
+    int64_t launches[TK_COUNT] = {};
+    void reset();
+
+  private:
+    struct Pending
+    {
+        hipEvent_t a, b;
+        int kernel, iter;
+    };
+    std::vector<Pending> pending_;
+    std::vector<hipEvent_t> pool_;
+    hipEvent_t get();
+};
+
+// Mesh: src/Mesh.hpp:32-145.
+class Mesh
+{
+  public:
+    explicit Mesh(const cf_config& c);
+    const Geo& geo() const { return _geo; }
+    double cellSize() const { return _geo.dx; }
+    std::array<int, 3> minDomainGlobalCellIndex() const { return {0, 0, 0}; }
+    std::array<int, 3> maxDomainGlobalCellIndex() const
+    {
+        return {_geo.n[0] - 1, _geo.n[1] - 1, _geo.n[2] - 1};
+    }
+
+  private:
+    Geo _geo;
+};
+
+// ProblemManager: src/ProblemManager.hpp:97-418.  Current/Next versions of
+// q, u, v (w) with zeroed ghosts; advance swaps; gather fills ghosts.
+class ProblemManager
+{
+  public:
+    ProblemManager(const Mesh& mesh, hipStream_t s);
+    double* get(int entity, int version) const
+    {
+        return _f[entity][_cur[entity] ^ version]->ptr;
+    }
+    Fields fields(int version) const;
+    void advance(int entity) { _cur[entity] ^= 1; }
+    void gather(int version);
+    const Mesh& mesh() const { return _mesh; }
+
+  private:
+    const Mesh& _mesh;
+    hipStream_t _s;
+    std::unique_ptr<DevArray<double>> _f[4][2];
+    int _cur[4] = {0, 0, 0, 0};
+};
+
+// VelocityCorrector: src/VelocityCorrector.hpp:50-295 with the Jacobi-PCG of
+// the "Reference" solver choice run as matrix-free HIP kernels.
+class VelocityCorrector
+{
+  public:
+    VelocityCorrector(ProblemManager& pm, const cf_config& cfg, hipStream_t s,
+                      KernelTimer& timer);
+    ~VelocityCorrector();
+    cf_status correctVelocity(cf_solve_stats* st);
+    void buildRHS();
+    void removeMean();
+    cf_status solve(int precision, const double* b, double* x,
+                    cf_solve_stats* st);
+    void applyPressure();
+    double* lhs() const { return _lhs.ptr; }
+    double* rhs() const { return _rhs.ptr; }
+
+  private:
+    cf_status solve64(const double* b, double* x, cf_solve_stats* st);
+    cf_status solveRefined(const double* b, double* x, cf_solve_stats* st);
+    int runPcg64(const double* b, double* x, double tol, int maxit);
+    int runPcg32(double tol, int maxit);
+    void readState(PcgState& host);
+    void ensure32();
+
+    ProblemManager& _pm;
+    cf_config _cfg;
+    hipStream_t _s;
+    KernelTimer& _timer;
+    const Geo& _g;
+    DevArray<double> _lhs, _rhs, _r, _p, _q;
+    DevArray<float> _d32, _r32, _p32, _q32;
+    DevArray<double> _red_partials, _scratch;
+    DevArray<unsigned> _red_counter;
+    PcgState* _st = nullptr;       // device
+    RefineState* _rs = nullptr;    // device
+    PcgState* _st_host = nullptr;  // pinned
+    RefineState* _rs_host = nullptr;
+    int _predict64 = 0, _predict32 = 0;
+};
+
+// Solver: src/Solver.hpp:54-279 (setup / step / _addInputs; the dt clamp is
+// applied when the Mesh geometry is built).
+class Solver
+{
+  public:
+    explicit Solver(const cf_config& cfg);
+    ~Solver();
+    cf_status setup(cf_solve_stats* st);
+    cf_status step(cf_solve_stats* st);
+    cf_status run(int nsteps, cf_solve_stats* last);
+    void timeIntegrate();
+    void addInputs();
+
+    const cf_config& config() const { return _cfg; }
+    const Geo& geo() const { return _mesh->geo(); }
+    hipStream_t stream() const { return _s; }
+    ProblemManager& pm() { return *_pm; }
+    VelocityCorrector& vc() { return *_vc; }
+    KernelTimer& timer() { return _timer; }
+    double time() const { return _time; }
+    std::string last_error;
+
+    void initVortexTubes(uint64_t seed, double peak);
+    void initBlobs(uint64_t seed, int nblobs);
+    void initRandomFaces(uint64_t seed);
+
+  private:
+    cf_config _cfg;
+    hipStream_t _s = nullptr;
+    KernelTimer _timer;
+    std::unique_ptr<Mesh> _mesh;
+    std::unique_ptr<ProblemManager> _pm;
+    std::unique_ptr<VelocityCorrector> _vc;
+    DevArray<unsigned long long> _vmax;
+    double _time = 0.0;
+};
+
+// TimeIntegrator::step (src/TimeIntegrator.hpp:120-177): gather, advect all
+// entities into Next in one launch, swap.
+namespace TimeIntegrator
+{
+void step(ProblemManager& pm, hipStream_t s, KernelTimer& timer);
+}
+
+} // namespace cfmi

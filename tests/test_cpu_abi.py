@@ -1,0 +1,91 @@
+"""Host-only checks of the C ABI: the library loads, exports every entry
+point include/cf_api.h declares, and its geometry queries reproduce the
+reference's mesh pins (tests/tstMesh.cpp:16-69) and dt clamp
+(src/Solver.hpp:93-106).  No device is touched."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from cajitafluids_amd import _lib
+from cajitafluids_amd import configs as K
+from cajitafluids_amd.solver import geometry, layout
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(REPO, "include", "cf_api.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cf_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.load()
+    syms = declared_symbols()
+    assert len(syms) >= 30
+    for s in syms:
+        assert hasattr(L, s), s
+        assert s in _lib.SIGNATURES, f"{s} missing from the ctypes table"
+
+
+def test_mesh_pins_512():
+    # MeshTest: 512 x 512 unit square, halo 3 (tests/tstMesh.hpp:44-57)
+    cfg = K.base_config(2, 512)
+    dx, _ = geometry(cfg)
+    assert dx == 1.0 / 512
+    ext, lo, hi = layout(cfg, K.CELL)
+    assert (hi[0] - lo[0], hi[1] - lo[1]) == (512, 512)
+    assert (lo[0] - 3, hi[0] - 1 - 3) == (0, 511)     # global cell index 0 .. N-1
+    ext, lo, hi = layout(cfg, K.FACE_X)
+    assert ext[:2] == (512 + 2 * 3 + 1, 512 + 2 * 3)  # tstMesh.cpp:61-68
+    assert (hi[0] - lo[0], hi[1] - lo[1]) == (513, 512)
+
+
+def test_layout_3d_and_periodic():
+    cfg = K.periodic_box_3d(24)
+    ext, lo, hi = layout(cfg, K.FACE_X)
+    assert ext == (24 + 2 * 2 + 1, 24 + 4, 24 + 4)
+    assert hi[0] - lo[0] == 24           # periodic: duplicated face owned once
+    ext, lo, hi = layout(cfg, K.FACE_Z)
+    assert hi[2] - lo[2] == 25           # wall axis: both walls owned
+
+
+def test_dt_clamp_configs():
+    # Solver.hpp:96-106: dt = dx / max|inflow v| when larger
+    _, dt = geometry(K.dye_tank_2d(128))
+    assert dt == (1.0 / 128) / 3.0
+    _, dt = geometry(K.vortex_tank_3d(256))
+    assert dt == (1.0 / 256) / 2.0
+    c = K.dye_tank_2d(128)
+    c["clamp_dt"] = 0
+    assert geometry(c)[1] == 0.005
+
+
+@pytest.mark.parametrize("bad", [
+    dict(dim=4), dict(halo=0), dict(field_order=2), dict(density=0.0),
+    dict(num_cell=[0, 8, 1]), dict(hi=[1.0, 0.7, 1.0]),
+])
+def test_invalid_configs_rejected_before_any_launch(bad):
+    cfg = K.base_config(2, 8)
+    cfg.update(bad)
+    with pytest.raises(_lib.CfError) as ei:
+        layout(cfg, K.CELL)
+    assert ei.value.code == _lib.CF_EINVAL
+
+
+def test_create_without_device_fails_cleanly():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except Exception:
+        pass
+    L = _lib.load()
+    st = _lib.config_struct(K.base_config(2, 8))
+    h = C.c_void_p()
+    rc = L.cf_create(C.byref(st), C.byref(h))
+    assert rc == _lib.CF_EDEVICE
+    assert not h.value
+    assert L.cf_last_error(None)

@@ -1,0 +1,233 @@
+"""Parity of the HIP path against the CPU oracle, through the C ABI.
+
+The oracle and the kernels evaluate every expression in the same order with
+the same explicit fma() calls, and their dot products are double-double, so
+the expected difference is zero; the asserted bar is the hot path's 1e-12
+relative L2 (FP64) with identical PCG iteration counts.
+"""
+import numpy as np
+import pytest
+
+from cajitafluids_amd import configs as K
+from helpers import compare_state, gpu_to_oracle, rel_l2, small_2d, small_3d
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def cf():
+    from cajitafluids_amd import solver
+    return solver
+
+
+def _pair(cf, oracle_mod, cfg):
+    sol = cf.Solver(cfg)
+    orc = oracle_mod.Oracle(cfg)
+    return sol, orc
+
+
+def _assert_state(sol, orc, tol=TOL, what=""):
+    res = compare_state(sol, orc)
+    for e, (rl2, mx) in res.items():
+        assert rl2 <= tol, f"{what} entity {e}: rel L2 {rl2:.3e} (max abs {mx:.3e})"
+
+
+@pytest.mark.parametrize("dim,n", [(2, 32), (3, 12)])
+def test_pcg_fp64_random_rhs(cf, oracle_mod, dim, n):
+    cfg = K.base_config(dim, n)
+    cfg.update(clamp_dt=0, dt=0.002)
+    sol, orc = _pair(cf, oracle_mod, cfg)
+    sol.init_random_faces(3)
+    gpu_to_oracle(sol, orc)
+    sol.build_rhs()
+    sol.rhs_remove_mean()
+    orc.build_rhs()
+    orc.rhs_remove_mean()
+    b_g = sol.owned_view(sol.get_pressure(1), 0)
+    b_o = orc.owned_view(orc.rhs(), 0)
+    assert rel_l2(b_g, b_o) <= 1e-15
+    st_g = sol.pressure_solve(K.FP64)
+    rc, st_o = orc.pressure_solve(K.FP64)
+    assert rc == 0 and st_g["converged"]
+    assert st_g["iterations"] == st_o["iterations"]
+    x_g = sol.owned_view(sol.get_pressure(0), 0)
+    x_o = orc.owned_view(orc.lhs(), 0)
+    assert rel_l2(x_g, x_o) <= TOL
+    assert st_g["final_residual"] < cfg["cg_tol"]
+
+
+@pytest.mark.parametrize("dim,n", [(2, 32), (3, 12)])
+def test_pcg_fp32_refined_random_rhs(cf, oracle_mod, dim, n):
+    cfg = K.base_config(dim, n)
+    cfg.update(clamp_dt=0, dt=0.002, precision=K.FP32_REFINED)
+    sol, orc = _pair(cf, oracle_mod, cfg)
+    sol.init_random_faces(3)
+    gpu_to_oracle(sol, orc)
+    sol.build_rhs()
+    sol.rhs_remove_mean()
+    orc.build_rhs()
+    orc.rhs_remove_mean()
+    st_g = sol.pressure_solve(K.FP32_REFINED)
+    rc, st_o = orc.pressure_solve(K.FP32_REFINED)
+    assert rc == 0 and st_g["converged"]
+    assert st_g["iterations"] == st_o["iterations"]
+    assert st_g["outer_iterations"] == st_o["outer_iterations"]
+    assert st_g["final_residual"] < 1e-8
+    x_g = sol.owned_view(sol.get_pressure(0), 0)
+    x_o = orc.owned_view(orc.lhs(), 0)
+    assert rel_l2(x_g, x_o) <= TOL
+
+
+def test_config1_small_steps(cf, oracle_mod):
+    cfg = small_2d(32)
+    sol, orc = _pair(cf, oracle_mod, cfg)
+    assert sol.dt == orc.dt
+    st = sol.setup()
+    rc, so = orc.setup()
+    assert rc == 0 and st["iterations"] == so["iterations"]
+    _assert_state(sol, orc, what="setup")
+    for n in range(5):
+        st = sol.step()
+        rc, so = orc.step()
+        assert rc == 0 and st["iterations"] == so["iterations"], n
+        _assert_state(sol, orc, what=f"step {n}")
+
+
+def test_config1_full(cf, oracle_mod):
+    """Config 1 exactly: 2-D 128^2, linear fields, 50 RK3 steps."""
+    cfg = K.dye_tank_2d(128)
+    sol, orc = _pair(cf, oracle_mod, cfg)
+    sol.setup()
+    orc.setup()
+    its_g, its_o = [], []
+    for _ in range(K.CONFIG1_STEPS):
+        its_g.append(sol.step()["iterations"])
+        rc, so = orc.step()
+        assert rc == 0
+        its_o.append(so["iterations"])
+    assert its_g == its_o
+    _assert_state(sol, orc, what="config1 final")
+
+
+def test_config1_cubic_as_written(cf, oracle_mod):
+    """The as-written field order (cubic) on config 1's setup at 64^2."""
+    cfg = small_2d(64, field_order=3)
+    sol, orc = _pair(cf, oracle_mod, cfg)
+    sol.setup()
+    orc.setup()
+    for _ in range(5):
+        sol.step()
+        orc.step()
+    _assert_state(sol, orc, what="cubic")
+
+
+@pytest.mark.parametrize("quirks", [K.QUIRK_RK3_V0, 0, K.QUIRK_RK3_V0 | K.QUIRK_PROJ_BC_U])
+def test_3d_vortex_steps(cf, oracle_mod, quirks):
+    cfg = small_3d(16, quirks=quirks)
+    sol, orc = _pair(cf, oracle_mod, cfg)
+    sol.init_vortex_tubes(K.CONFIG2_SEED, 1.0)
+    gpu_to_oracle(sol, orc)
+    sol.setup()
+    orc.setup()
+    _assert_state(sol, orc, what="3d setup")
+    for n in range(3):
+        st = sol.step()
+        rc, so = orc.step()
+        assert rc == 0 and st["iterations"] == so["iterations"]
+        _assert_state(sol, orc, what=f"3d step {n}")
+
+
+def test_3d_periodic_fp32_refined(cf, oracle_mod):
+    """Config 5's shape at 16^3: h = 2, periodic x/y, walls in z, cubic,
+    FP32-refined pressure, vortex tubes + blob dye."""
+    cfg = K.periodic_box_3d(16)
+    sol, orc = _pair(cf, oracle_mod, cfg)
+    sol.init_vortex_tubes(K.CONFIG5_VEL_SEED, 1.0)
+    sol.init_blobs(K.CONFIG5_DYE_SEED, 8)
+    gpu_to_oracle(sol, orc)
+    sol.setup()
+    orc.setup()
+    for n in range(3):
+        st = sol.step()
+        rc, so = orc.step()
+        assert rc == 0
+        assert st["iterations"] == so["iterations"]
+        assert st["outer_iterations"] == so["outer_iterations"]
+        _assert_state(sol, orc, what=f"periodic step {n}")
+
+
+def test_init_generators_match_oracle(cf, oracle_mod):
+    for cfg in (small_3d(16), K.periodic_box_3d(16), small_2d(32)):
+        sol, orc = _pair(cf, oracle_mod, cfg)
+        sol.init_vortex_tubes(5, 1.0)
+        orc.init_vortex_tubes(5, 1.0)
+        for e in range(1, cfg["dim"] + 1):
+            g = sol.owned_view(sol.get(e), e)
+            o = orc.owned_view(orc.field(e), e)
+            assert rel_l2(g, o) <= 1e-13
+        sol.init_blobs(55, 8)
+        qo = orc.init_blobs(55, 8)
+        assert rel_l2(sol.owned_view(sol.get(0), 0), orc.owned_view(qo, 0)) <= 1e-13
+
+
+def test_ghost_fill_periodic(cf, oracle_mod):
+    cfg = K.periodic_box_3d(12)
+    sol, orc = _pair(cf, oracle_mod, cfg)
+    rng = np.random.default_rng(0)
+    for e in range(4):
+        a = np.zeros(sol.extents(e)[::-1])
+        lo, hi = sol.owned(e)
+        a[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]] = rng.standard_normal(
+            (hi[2] - lo[2], hi[1] - lo[1], hi[0] - lo[0]))
+        sol.set(e, a)
+        orc.field(e)[...] = a
+    sol.gather()
+    orc.fill_ghosts()
+    for e in range(4):
+        assert np.array_equal(sol.get(e), np.array(orc.field(e)))
+
+
+def test_reference_halo_semantics_single_rank(cf):
+    """tests/tstProblemManager.cpp:61-97 on one rank with walls: no neighbour,
+    so gather leaves every ghost untouched (zero)."""
+    cfg = K.base_config(2, 16)
+    sol = cf.Solver(cfg)
+    ex = sol.extents(K.FACE_X)
+    lo, hi = sol.owned(K.FACE_X)
+    a = np.zeros(ex[::-1])
+    for j in range(lo[1], hi[1]):
+        for i in range(lo[0], hi[0]):
+            a[0, j, i] = i * 100 + j * 10 + 1
+    sol.set(K.FACE_X, a)
+    sol.gather()
+    assert np.array_equal(sol.get(K.FACE_X), a)
+
+
+def test_advance_swaps(cf):
+    """tests/tstProblemManager.cpp:23-59: advance makes Next current."""
+    cfg = K.base_config(2, 16)
+    sol = cf.Solver(cfg)
+    ex = sol.extents(K.CELL)
+    lo, hi = sol.owned(K.CELL)
+    cur = np.zeros(ex[::-1])
+    nxt = np.zeros(ex[::-1])
+    for j in range(lo[1], hi[1]):
+        for i in range(lo[0], hi[0]):
+            cur[0, j, i] = i * 100 + j * 10
+            nxt[0, j, i] = i * 100 + j * 10 + 5
+    sol.set(K.CELL, cur, 0)
+    sol.set(K.CELL, nxt, 1)
+    sol.advance(K.CELL)
+    assert np.array_equal(sol.get(K.CELL, 0), nxt)
+
+
+def test_nonconvergence_reports_enoconv(cf):
+    from cajitafluids_amd import _lib
+    cfg = small_2d(32, cg_max_iter=3)
+    sol = cf.Solver(cfg)
+    with pytest.raises(_lib.CfError) as ei:
+        sol.setup()
+    assert ei.value.code == _lib.CF_ENOCONV
+    assert sol.last_stats["iterations"] == 3
