@@ -20,10 +20,13 @@ sys.path.insert(0, REPO)
 METRIC = "cell-steps/s at 256^3 + PCG HBM GB/s; FP32-refined vs FP64 at 1e-8 resid"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes per owned cell per launch of each PCG kernel (FP64
-# words x 8; FP32 words x 4): spmv+dot reads p, writes q; update reads x, p,
-# r, q and writes x, r; direction reads r, p and writes p.
-WORDS = {"spmv": 2, "update": 6, "dir": 3}
+# Algorithmic vector words per owned cell per launch of each PCG kernel
+# (x 8 bytes in FP64): dir_spmv reads r and the old direction and writes the
+# new one; update reads x, p, r and writes x, r.  The reference-shaped split
+# (SpMV / update / direction) moves 11 words per iteration; the fused pair
+# moves 8.  "pcg.gbs_11_words" counts 11 words per iteration (the contract's
+# convention), the roofline object counts each kernel's own words.
+WORDS = {"dir_spmv": 3, "update": 5}
 
 
 def parse():
@@ -87,14 +90,15 @@ def main():
     del sol
 
     value = ncell * args.steps / t64
-    names = {_lib.TK_SPMV: "spmv", _lib.TK_UPDATE: "update", _lib.TK_DIR: "dir"}
+    names = {_lib.TK_DIR_SPMV: "dir_spmv", _lib.TK_UPDATE: "update"}
     per = {}
     for k, nm in names.items():
         ms, cnt = kt[k]
         if cnt:
             per[nm] = dict(avg_us=1e3 * ms / cnt, launches=cnt, total_ms=ms,
                            bytes=WORDS[nm] * 8 * ncell)
-    t_pcg = sum(p["total_ms"] for p in per.values()) / 1e3
+    fin_ms, fin_n = kt[_lib.TK_FINALIZE]
+    t_pcg = (sum(p["total_ms"] for p in per.values()) + fin_ms) / 1e3
     total_iters = sum(it64)
     pcg_gbs = 88.0 * ncell * total_iters / t_pcg / 1e9 if t_pcg > 0 else None
     dom = max(per, key=lambda k: per[k]["total_ms"])
@@ -126,6 +130,7 @@ def main():
                 "iterations": it64, "gbs_11_words": round(pcg_gbs, 1) if pcg_gbs else None,
                 "frac_11_words": round(pcg_gbs / HBM_PEAK_GBS, 4) if pcg_gbs else None,
                 "time_share": round(t_pcg / t64, 4),
+                "finalize_avg_us": round(1e3 * fin_ms / fin_n, 2) if fin_n else None,
                 "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv)
                                 for kk, vv in v.items()} for k, v in per.items()},
                 "max_final_residual": max(res64)},

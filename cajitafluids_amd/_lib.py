@@ -15,7 +15,8 @@ STATUS_NAMES = {0: "CF_OK", 1: "CF_EINVAL", 2: "CF_ENOMEM", 3: "CF_ENOCONV",
                 4: "CF_EDEVICE", 5: "CF_ENOTIMPL"}
 
 # kernel ids of cf_kernel_time
-TK_SPMV, TK_UPDATE, TK_DIR, TK_SPMV32, TK_UPDATE32, TK_DIR32, TK_ADVECT, TK_RESIDUAL = range(8)
+(TK_DIR_SPMV, TK_UPDATE, TK_FINALIZE, TK_DIR_SPMV32, TK_UPDATE32, TK_FINALIZE32,
+ TK_ADVECT, TK_RESIDUAL) = range(8)
 
 
 class CfConfig(C.Structure):

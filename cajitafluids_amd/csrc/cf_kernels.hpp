@@ -8,27 +8,26 @@
 namespace cfmi
 {
 
-// Device-resident state of one PCG solve.  The last block of each reducing
-// kernel computes the scalars, so the host never waits on a reduction; it
-// polls `done` once per batch of iterations.
+// Device-resident state of one PCG solve.  Finalize kernels update the
+// scalars on the device, so the host never waits on a reduction; it polls
+// `done` once per batch of iterations.
 struct PcgState
 {
     double rho, alpha, beta, bnorm, res, tol;
     int iter, maxit, done, converged;
 };
 
-// Scratch of a grid reduction: per-block double-double partials and the
-// arrival counter (wraps back to 0 after every launch).
+// Grid-reduction scratch of the streaming kernels: per-block double-double
+// partials (kRedBlocks * 2 values * 2 doubles) and the arrival counter,
+// which the last block resets to 0.
 struct RedBuf
 {
-    double* partials; // kRedBlocks * 4 * 2 doubles
+    double* partials;
     unsigned* counter;
 };
 
 constexpr int kBlock = 256;
 constexpr int kRedBlocks = 2048; // grid of the streaming kernels (multiple of 8)
-
-int stream_grid(const Geo& g);
 
 // State arrays of one version: q, u, v, w (w null in 2-D).
 struct Fields
@@ -46,17 +45,22 @@ void launch_rhs_shift(const Geo& g, double* rhs, const double* sum,
                       hipStream_t s);
 void launch_project(const Geo& g, Fields cur, const double* p, hipStream_t s);
 
-// FP64 Jacobi-PCG (x = 0 start).
-void launch_pcg_init(const Geo& g, const double* b, double* x, double* r,
-                     double* p, PcgState* st, double tol, int maxit,
-                     RedBuf rb, hipStream_t s);
-void launch_pcg_spmv_dot(const Geo& g, const double* p, double* q,
-                         PcgState* st, RedBuf rb, hipStream_t s);
-void launch_pcg_update(const Geo& g, double* x, double* r, const double* p,
-                       const double* q, PcgState* st, RedBuf rb,
+// Jacobi-PCG (cf_pcg.hip).  init64: r = b, x = 0; init32: d = 0 for the
+// FP32 inner solve of the residual already in r.
+void launch_pcg_init64(const Geo& g, const double* b, double* x, double* r,
+                       PcgState* st, double tol, int maxit, RedBuf rb,
                        hipStream_t s);
-void launch_pcg_dir(const Geo& g, const double* r, double* p,
-                    const PcgState* st, hipStream_t s);
+void launch_pcg_init32(const Geo& g, const float* r, float* d, PcgState* st,
+                       double tol, int maxit, RedBuf rb, hipStream_t s);
+template <class T>
+void launch_pcg_dir_spmv(const Geo& g, const T* r, const T* p_old, T* p_new,
+                         PcgState* st, RedBuf rb, hipStream_t s);
+template <class T>
+void launch_pcg_update(const Geo& g, T* x, T* r, const T* p, PcgState* st,
+                       RedBuf rb, hipStream_t s);
+// Largest grid any reducing kernel of this geometry launches (sizes the
+// partials buffer).
+int reduction_blocks_max(const Geo& g);
 
 // FP32-inner / FP64-outer defect correction.  The host runs the outer loop
 // from the device-computed norms (one read-back per pass).
@@ -71,15 +75,5 @@ void launch_refine_add(const Geo& g, double* x64, const float* d32,
 void launch_refine_residual(const Geo& g, const double* b, const double* x64,
                             float* r32, RefineState* rs, RedBuf rb,
                             hipStream_t s);
-void launch_pcg32_init(const Geo& g, const float* r, float* d, float* p,
-                       PcgState* st, double tol, int maxit, RedBuf rb,
-                       hipStream_t s);
-void launch_pcg32_spmv_dot(const Geo& g, const float* p, float* q,
-                           PcgState* st, RedBuf rb, hipStream_t s);
-void launch_pcg32_update(const Geo& g, float* d, float* r, const float* p,
-                         const float* q, PcgState* st, RedBuf rb,
-                         hipStream_t s);
-void launch_pcg32_dir(const Geo& g, const float* r, float* p,
-                      const PcgState* st, hipStream_t s);
 
 } // namespace cfmi

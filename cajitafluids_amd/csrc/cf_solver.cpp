@@ -178,9 +178,9 @@ VelocityCorrector::VelocityCorrector(ProblemManager& pm, const cf_config& cfg,
     _lhs.alloc(n, s);
     _rhs.alloc(n, s);
     _r.alloc(n, s);
-    _p.alloc(n, s);
-    _q.alloc(n, s);
-    _red_partials.alloc((int64_t)kRedBlocks * 4 * 2, s);
+    _p[0].alloc(n, s);
+    _p[1].alloc(n, s);
+    _red_partials.alloc((int64_t)reduction_blocks_max(_g) * 2 * 2, s);
     _red_counter.alloc(1, s);
     _scratch.alloc(8, s);
     void* p = nullptr;
@@ -217,8 +217,8 @@ void VelocityCorrector::ensure32()
     const int64_t n = _g.total;
     _d32.alloc(n, _s);
     _r32.alloc(n, _s);
-    _p32.alloc(n, _s);
-    _q32.alloc(n, _s);
+    _p32[0].alloc(n, _s);
+    _p32[1].alloc(n, _s);
 }
 
 void VelocityCorrector::readState(PcgState& host)
@@ -251,34 +251,29 @@ void VelocityCorrector::applyPressure()
     launch_project(_g, _pm.fields(CF_CURRENT), _lhs.ptr, _s);
 }
 
-// Launch batches of iterations without waiting; poll the device flag once
+// Queue batches of iterations without waiting; poll the device flag once
 // per batch.  The first batch is sized from the previous solve's count so
-// that few no-op launches follow convergence.
-int VelocityCorrector::runPcg64(const double* b, double* x, double tol,
-                                int maxit)
+// that few no-op launches follow convergence.  p[0]/p[1] alternate as the
+// old and new direction.
+template <class T>
+int VelocityCorrector::runIterations(T* x, T* r, DevArray<T> (&p)[2],
+                                     int& predict, int tk0)
 {
     RedBuf rb{_red_partials.ptr, _red_counter.ptr};
-    launch_pcg_init(_g, b, x, _r.ptr, _p.ptr, _st, tol, maxit, rb, _s);
     PcgState hs;
-    if (maxit <= 0) {
-        readState(hs);
-        return hs.iter;
-    }
     int launched = 0;
-    int batch = _predict64 > 4 ? _predict64 - 2 : 16;
+    int cur = 0;
+    int batch = predict > 4 ? predict - 2 : 16;
     for (;;) {
         for (int c = 0; c < batch; ++c) {
             const int it = launched + c + 1;
             int t0 = _timer.begin(_s);
-            launch_pcg_spmv_dot(_g, _p.ptr, _q.ptr, _st, rb, _s);
-            _timer.end(t0, _s, TK_SPMV, it);
-            int t1 = _timer.begin(_s);
-            launch_pcg_update(_g, x, _r.ptr, _p.ptr, _q.ptr, _st, rb, _s);
-            _timer.end(t1, _s, TK_UPDATE, it);
+            launch_pcg_dir_spmv<T>(_g, r, p[cur].ptr, p[cur ^ 1].ptr, _st, rb, _s);
+            _timer.end(t0, _s, tk0 + 0, it);
             int t2 = _timer.begin(_s);
-            launch_pcg_dir(_g, _r.ptr, _p.ptr, _st, _s);
-            // The direction update of the last iteration is a no-op.
-            _timer.end(t2, _s, TK_DIR, it + 1);
+            launch_pcg_update<T>(_g, x, r, p[cur ^ 1].ptr, _st, rb, _s);
+            _timer.end(t2, _s, tk0 + 1, it);
+            cur ^= 1;
         }
         launched += batch;
         readState(hs);
@@ -287,8 +282,20 @@ int VelocityCorrector::runPcg64(const double* b, double* x, double tol,
             break;
         batch = 4;
     }
-    _predict64 = hs.iter;
+    predict = hs.iter;
     return hs.iter;
+}
+
+int VelocityCorrector::runPcg64(const double* b, double* x, double tol,
+                                int maxit)
+{
+    RedBuf rb{_red_partials.ptr, _red_counter.ptr};
+    launch_pcg_init64(_g, b, x, _r.ptr, _st, tol, maxit, rb, _s);
+    PcgState hs;
+    readState(hs);
+    if (hs.done)
+        return hs.iter;
+    return runIterations<double>(x, _r.ptr, _p, _predict64, TK_DIR_SPMV);
 }
 
 cf_status VelocityCorrector::solve64(const double* b, double* x,
@@ -309,37 +316,13 @@ cf_status VelocityCorrector::solve64(const double* b, double* x,
 int VelocityCorrector::runPcg32(double tol, int maxit)
 {
     RedBuf rb{_red_partials.ptr, _red_counter.ptr};
-    launch_pcg32_init(_g, _r32.ptr, _d32.ptr, _p32.ptr, _st, tol, maxit, rb, _s);
+    launch_pcg_init32(_g, _r32.ptr, _d32.ptr, _st, tol, maxit, rb, _s);
     PcgState hs;
-    if (maxit <= 0) {
-        readState(hs);
-        return 0;
-    }
-    int launched = 0;
-    int batch = _predict32 > 4 ? _predict32 - 2 : 16;
-    for (;;) {
-        for (int c = 0; c < batch; ++c) {
-            const int it = launched + c + 1;
-            int t0 = _timer.begin(_s);
-            launch_pcg32_spmv_dot(_g, _p32.ptr, _q32.ptr, _st, rb, _s);
-            _timer.end(t0, _s, TK_SPMV32, it);
-            int t1 = _timer.begin(_s);
-            launch_pcg32_update(_g, _d32.ptr, _r32.ptr, _p32.ptr, _q32.ptr, _st,
-                                rb, _s);
-            _timer.end(t1, _s, TK_UPDATE32, it);
-            int t2 = _timer.begin(_s);
-            launch_pcg32_dir(_g, _r32.ptr, _p32.ptr, _st, _s);
-            _timer.end(t2, _s, TK_DIR32, it + 1);
-        }
-        launched += batch;
-        readState(hs);
-        _timer.resolve(hs.done ? hs.iter + 1 : INT_MAX);
-        if (hs.done)
-            break;
-        batch = 4;
-    }
-    _predict32 = hs.iter;
-    return hs.iter;
+    readState(hs);
+    if (hs.done)
+        return hs.iter;
+    return runIterations<float>(_d32.ptr, _r32.ptr, _p32, _predict32,
+                                TK_DIR_SPMV32);
 }
 
 cf_status VelocityCorrector::solveRefined(const double* b, double* x,

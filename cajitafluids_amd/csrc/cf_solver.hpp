@@ -54,14 +54,14 @@ struct DevArray
 // Per-kernel device time from HIP events on the handle's stream.
 enum TimedKernel
 {
-    TK_SPMV = 0,
-    TK_UPDATE = 1,
-    TK_DIR = 2,
-    TK_SPMV32 = 3,
+    TK_DIR_SPMV = 0,   // k_pcg_dir_spmv<double>
+    TK_UPDATE = 1,     // k_pcg_update<double>
+    TK_FINALIZE = 2,   // the two one-block finalize kernels of an iteration
+    TK_DIR_SPMV32 = 3, // FP32 inner solve
     TK_UPDATE32 = 4,
-    TK_DIR32 = 5,
+    TK_FINALIZE32 = 5,
     TK_ADVECT = 6,
-    TK_RESIDUAL = 7,
+    TK_RESIDUAL = 7,   // FP64 defect-correction residual
     TK_COUNT = 8
 };
 
@@ -150,6 +150,8 @@ class VelocityCorrector
   private:
     cf_status solve64(const double* b, double* x, cf_solve_stats* st);
     cf_status solveRefined(const double* b, double* x, cf_solve_stats* st);
+    template <class T>
+    int runIterations(T* x, T* r, DevArray<T> (&p)[2], int& predict, int tk0);
     int runPcg64(const double* b, double* x, double tol, int maxit);
     int runPcg32(double tol, int maxit);
     void readState(PcgState& host);
@@ -160,8 +162,8 @@ class VelocityCorrector
     hipStream_t _s;
     KernelTimer& _timer;
     const Geo& _g;
-    DevArray<double> _lhs, _rhs, _r, _p, _q;
-    DevArray<float> _d32, _r32, _p32, _q32;
+    DevArray<double> _lhs, _rhs, _r, _p[2];
+    DevArray<float> _d32, _r32, _p32[2];
     DevArray<double> _red_partials, _scratch;
     DevArray<unsigned> _red_counter;
     PcgState* _st = nullptr;       // device

@@ -177,8 +177,9 @@ cf_status cf_init_random_faces(cf_solver* h, uint64_t seed);
 /* When enabled, the PCG brackets each launch of its kernels with HIP events
  * on the handle's stream and accumulates per-kernel device time. */
 cf_status cf_set_kernel_timing(cf_solver* h, int enable);
-/* kernel: 0 spmv/direction kernel, 1 update kernel, 2 FP32 spmv/direction,
- * 3 FP32 update, 4 advect, 5 FP64 residual (defect correction) */
+/* kernel: 0 FP64 direction+SpMV, 1 FP64 update, 2 FP64 finalize kernels,
+ * 3-5 the same for the FP32 inner solve, 6 advection, 7 FP64
+ * defect-correction residual */
 cf_status cf_kernel_time(cf_solver* h, int kernel, double* total_ms,
                          int64_t* launches);
 cf_status cf_reset_kernel_time(cf_solver* h);

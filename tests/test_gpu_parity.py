@@ -80,6 +80,32 @@ def test_pcg_fp32_refined_random_rhs(cf, oracle_mod, dim, n):
     assert rel_l2(x_g, x_o) <= TOL
 
 
+@pytest.mark.parametrize("periodic", [[0, 0, 0], [1, 1, 0], [0, 1, 1]])
+@pytest.mark.parametrize("precision", [K.FP64, K.FP32_REFINED])
+def test_pcg_3d_ragged_tiles(cf, oracle_mod, periodic, precision):
+    """Non-cubic 70 x 36 x 40 grid: partial 64 x 16 tiles in x and y and
+    several z chunks, with walls or periodic axes."""
+    n = [70, 36, 40]
+    cfg = K.base_config(3, 8)
+    dx = 1.0 / 70
+    cfg.update(num_cell=n, hi=[1.0, 36 * dx, 40 * dx], clamp_dt=0, dt=0.002,
+               periodic=periodic, halo=2, precision=precision)
+    sol, orc = _pair(cf, oracle_mod, cfg)
+    sol.init_random_faces(7)
+    gpu_to_oracle(sol, orc)
+    sol.build_rhs()
+    sol.rhs_remove_mean()
+    orc.build_rhs()
+    orc.rhs_remove_mean()
+    st_g = sol.pressure_solve(precision)
+    rc, st_o = orc.pressure_solve(precision)
+    assert rc == 0 and st_g["converged"]
+    assert st_g["iterations"] == st_o["iterations"]
+    x_g = sol.owned_view(sol.get_pressure(0), 0)
+    x_o = orc.owned_view(orc.lhs(), 0)
+    assert rel_l2(x_g, x_o) <= TOL
+
+
 def test_config1_small_steps(cf, oracle_mod):
     cfg = small_2d(32)
     sol, orc = _pair(cf, oracle_mod, cfg)
