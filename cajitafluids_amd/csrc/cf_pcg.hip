@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 
 #include "cf_stream.hpp"
+#include "cf_tile3.hpp"
 
 namespace cfmi
 {
@@ -177,188 +178,64 @@ __global__ __launch_bounds__(256) void k_pcg_update(
     });
 }
 
-// ---- 3-D: z-marching tiles ------------------------------------------------------
-// A block owns a 64 x 16 (x, y) tile and marches up a chunk of z planes.  Each
-// lane keeps its column's values at planes k-1, k, k+1 in registers (4 rows
-// per lane), and the plane k values of the tile plus a one-cell halo sit in
-// LDS for the x and y neighbours.  Every vector element is loaded once per
-// chunk (halo cells come from the neighbouring tiles, which the XCD-ordered
-// block mapping keeps in the same L2).
-namespace tile3
-{
-constexpr int TX = 64;          // lanes along x
-constexpr int NW = kBlock / TX; // waves per block
-constexpr int RY = 4;           // rows per lane
-constexpr int TY = NW * RY;     // tile rows
-constexpr int NHALO = 2 * TX + 2 * TY;
-} // namespace tile3
-
-struct Tile3Grid
-{
-    int tiles_x, tiles_y, nchunk, kz, work;
-};
-
-static Tile3Grid tile3_grid(const Geo& g)
-{
-    using namespace tile3;
-    Tile3Grid t;
-    t.tiles_x = (g.n[0] + TX - 1) / TX;
-    t.tiles_y = (g.n[1] + TY - 1) / TY;
-    t.kz = g.n[2] / 16 > 16 ? g.n[2] / 16 : 16;
-    t.nchunk = (g.n[2] + t.kz - 1) / t.kz;
-    t.work = t.tiles_x * t.tiles_y * t.nchunk;
-    return t;
-}
-
-static int tile3_blocks(const Geo& g)
-{
-    const Tile3Grid t = tile3_grid(g);
-    return (t.work + 7) / 8 * 8;
-}
-
-// Block -> (tile x, tile y, chunk): XCD x (= blockIdx mod 8) gets a
-// contiguous range of the x-fastest, then y, then chunk order.
-struct Tile3Pos
-{
-    int i0, j0, k0, k1;
-    bool active;
-};
-
-__device__ __forceinline__ Tile3Pos tile3_pos(const Geo& g, Tile3Grid t)
-{
-    using namespace tile3;
-    const int L = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
-    Tile3Pos p;
-    p.active = L < t.work;
-    const int tx = L % t.tiles_x;
-    const int rest = L / t.tiles_x;
-    const int ty = rest % t.tiles_y;
-    const int ch = rest / t.tiles_y;
-    p.i0 = tx * TX;
-    p.j0 = ty * TY;
-    p.k0 = ch * t.kz;
-    p.k1 = min(p.k0 + t.kz, g.n[2]);
-    return p;
-}
-
-// Maps a global index that may lie one cell outside the domain to the
-// stored cell it stands for: wrapped on a periodic axis; -1 (a zero ghost)
-// otherwise, and for anything further out.
-__device__ __forceinline__ int wrap_or_ghost(const Geo& g, int d, int gi)
-{
-    const int n = g.n[d];
-    if (gi >= 0 && gi < n)
-        return gi;
-    if (g.periodic[d] && (gi == -1 || gi == n))
-        return gi < 0 ? n - 1 : 0;
-    return -1;
-}
-
-// Jacobi diagonal inverse for a per-lane wall count (at most 6 walls).
+// ---- 3-D kernels (cf_tile3.hpp) ------------------------------------------------
 template <class T>
-__device__ __forceinline__ T dinv_sel(const Geo& g, int cnt)
+struct DirEpiT
 {
-    T v = dinv_of<T>(g, 0);
-#pragma unroll
-    for (int q = 1; q <= 6; ++q)
-        v = cnt == q ? dinv_of<T>(g, q) : v;
-    return v;
-}
-
-// New search direction at global cell (gi, gj, gk) (0 at a zero ghost).
-// UNIFORM_YZ: gj and gk are the same across the wave, so their part of the
-// wall count and the diagonal lookup stay scalar.
-template <class T, bool UNIFORM_YZ>
-__device__ __forceinline__ T pdir3(const Geo& g, const T* __restrict__ r,
-                                   const T* __restrict__ p_old, bool first,
-                                   T beta, int gi, int gj, int gk)
-{
-    const int a = wrap_or_ghost(g, 0, gi), b = wrap_or_ghost(g, 1, gj),
-              c = wrap_or_ghost(g, 2, gk);
-    if (a < 0 || b < 0 || c < 0)
-        return (T)0;
-    const int64_t at = g.idx(g.h + a, g.h + b, g.h + c);
-    T dinv;
-    if constexpr (UNIFORM_YZ)
-        dinv = dinv_pick<T>(g, wall_count(g, 1, b) + wall_count(g, 2, c),
-                            wall_count(g, 0, a));
-    else
-        dinv = dinv_sel<T>(g, wall_count(g, 0, a) + wall_count(g, 1, b) +
-                                  wall_count(g, 2, c));
-    const T z = dinv * r[at];
-    return first ? z : fma_t<T>(beta, p_old[at], z);
-}
-
-// Stored value at global cell (gi, gj, gk) (0 at a zero ghost).
-template <class T>
-__device__ __forceinline__ T load3(const Geo& g, const T* __restrict__ p, int gi,
-                                   int gj, int gk)
-{
-    const int a = wrap_or_ghost(g, 0, gi), b = wrap_or_ghost(g, 1, gj),
-              c = wrap_or_ghost(g, 2, gk);
-    if (a < 0 || b < 0 || c < 0)
-        return (T)0;
-    return p[g.idx(g.h + a, g.h + b, g.h + c)];
-}
-
-// Fills LDS plane `pl` (rows 0..TY+1, cols 0..TX+1) for the tile's cells
-// (vo, rows wave-uniform) and its x/y halo ring (vh) at plane gk; own[] gets
-// the lane's own cells.
-template <class T, class VO, class VH>
-__device__ __forceinline__ void tile3_fill(T (*pl)[tile3::TX + 2], int i0, int j0,
-                                           int gk, T (&own)[tile3::RY], VO&& vo,
-                                           VH&& v)
-{
-    using namespace tile3;
-    const int lane = threadIdx.x & (TX - 1);
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / TX);
-#pragma unroll
-    for (int q = 0; q < RY; ++q) {
-        const int jj = w * RY + q;
-        own[q] = vo(i0 + lane, j0 + jj, gk);
-        pl[jj + 1][lane + 1] = own[q];
+    const Geo& g;
+    T* __restrict__ p_new;
+    double hi[1] = {0.0}, lo[1] = {0.0};
+    __device__ void prefetch(const Tile3Ctx&, int) {}
+    __device__ void rotate() {}
+    __device__ void cell(const Tile3Ctx& c, int q, int, int, int k, T pc, T y)
+    {
+        p_new[c.colb[q] + g.px * g.py * (int64_t)(g.h + k)] = pc;
+        dd_add_prod(hi[0], lo[0], (double)pc, (double)y);
     }
-    for (int hh = threadIdx.x; hh < NHALO; hh += kBlock) {
-        int row, col, gi, gj;
-        if (hh < TX) {
-            row = 0; col = hh + 1; gi = i0 + hh; gj = j0 - 1;
-        } else if (hh < 2 * TX) {
-            row = TY + 1; col = hh - TX + 1; gi = i0 + hh - TX; gj = j0 + TY;
-        } else if (hh < 2 * TX + TY) {
-            row = hh - 2 * TX + 1; col = 0; gi = i0 - 1; gj = j0 + hh - 2 * TX;
-        } else {
-            row = hh - 2 * TX - TY + 1; col = TX + 1; gi = i0 + TX;
-            gj = j0 + hh - 2 * TX - TY;
+};
+
+template <class T>
+struct UpdEpiT
+{
+    const Geo& g;
+    T* __restrict__ x;
+    T* __restrict__ r;
+    T alpha;
+    double hi[2] = {0.0, 0.0}, lo[2] = {0.0, 0.0};
+    T xc[tile3::RY], rc[tile3::RY], xn[tile3::RY], rn_[tile3::RY];
+    __device__ void prefetch(const Tile3Ctx& c, int k)
+    {
+#pragma unroll
+        for (int q = 0; q < tile3::RY; ++q) {
+            const int gj = c.j0 + c.w * tile3::RY + q;
+            const bool ok = k < c.k1 && c.i0 + c.lane < g.n[0] && gj < g.n[1];
+            const int64_t at = c.colb[q] + g.px * g.py * (int64_t)(g.h + k);
+            xn[q] = ok ? x[at] : (T)0;
+            rn_[q] = ok ? r[at] : (T)0;
         }
-        pl[row][col] = v(gi, gj, gk);
     }
-}
-
-// y = A p at own cell (lane, row jj) of plane k: centre pc, z neighbours
-// pm / pp, x and y neighbours from the LDS plane; reference stencil order.
-template <class T>
-__device__ __forceinline__ T tile3_apply(const Geo& g, const T (*pl)[tile3::TX + 2],
-                                         int lane, int jj, int gi, int gj, int gk,
-                                         T pc, T pm, T pp)
-{
-    const T s = offdiag<T>(g);
-    auto cf = [&](int d, int side, int gidx) -> T {
-        if (g.periodic[d])
-            return s;
-        const bool wall = side == 0 ? gidx <= 0 : gidx > g.n[d] - 2;
-        return wall ? (T)0 : s;
-    };
-    T y = diag_pick<T>(g, wall_count(g, 1, gj) + wall_count(g, 2, gk),
-                       wall_count(g, 0, gi)) *
-          pc;
-    y = fma_t<T>(cf(0, 0, gi), pl[jj + 1][lane], y);
-    y = fma_t<T>(cf(0, 1, gi), pl[jj + 1][lane + 2], y);
-    y = fma_t<T>(cf(1, 0, gj), pl[jj][lane + 1], y);
-    y = fma_t<T>(cf(1, 1, gj), pl[jj + 2][lane + 1], y);
-    y = fma_t<T>(cf(2, 0, gk), pm, y);
-    y = fma_t<T>(cf(2, 1, gk), pp, y);
-    return y;
-}
+    __device__ void rotate()
+    {
+#pragma unroll
+        for (int q = 0; q < tile3::RY; ++q) {
+            xc[q] = xn[q];
+            rc[q] = rn_[q];
+        }
+    }
+    __device__ void cell(const Tile3Ctx& c, int q, int gi, int gj, int k, T pc,
+                         T qv)
+    {
+        const int64_t at = c.colb[q] + g.px * g.py * (int64_t)(g.h + k);
+        x[at] = fma_t<T>(alpha, pc, xc[q]);
+        const T rn = fma_t<T>(-alpha, qv, rc[q]);
+        r[at] = rn;
+        const T z = dinv_pick<T>(g, wall_count(g, 1, gj) + wall_count(g, 2, k),
+                                 wall_count(g, 0, gi)) *
+                    rn;
+        dd_add_prod(hi[0], lo[0], (double)rn, (double)rn);
+        dd_add_prod(hi[1], lo[1], (double)rn, (double)z);
+    }
+};
 
 template <class T>
 __global__ __launch_bounds__(256) void k_pcg_dir_spmv3(
@@ -371,45 +248,10 @@ __global__ __launch_bounds__(256) void k_pcg_dir_spmv3(
     __shared__ T pl[2][TY + 2][TX + 2];
     const bool first = st->iter == 0;
     const T beta = (T)st->beta;
-    double hi[1] = {0.0}, lo[1] = {0.0};
-    const Tile3Pos tp = tile3_pos(g, tg);
-    if (tp.active) {
-        const int lane = threadIdx.x & (TX - 1);
-        const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / TX);
-        const int gi = tp.i0 + lane;
-        auto po = [&](int a, int b, int c) {
-            return pdir3<T, true>(g, r, p_old, first, beta, a, b, c);
-        };
-        auto ph = [&](int a, int b, int c) {
-            return pdir3<T, false>(g, r, p_old, first, beta, a, b, c);
-        };
-        T pm[RY], pc[RY], pp[RY];
-#pragma unroll
-        for (int q = 0; q < RY; ++q)
-            pm[q] = po(gi, tp.j0 + w * RY + q, tp.k0 - 1);
-        tile3_fill<T>(pl[0], tp.i0, tp.j0, tp.k0, pc, po, ph);
-        __syncthreads();
-        int cur = 0;
-        for (int k = tp.k0; k < tp.k1; ++k) {
-            tile3_fill<T>(pl[cur ^ 1], tp.i0, tp.j0, k + 1, pp, po, ph);
-#pragma unroll
-            for (int q = 0; q < RY; ++q) {
-                const int jj = w * RY + q;
-                const int gj = tp.j0 + jj;
-                if (gi < g.n[0] && gj < g.n[1]) {
-                    const T y = tile3_apply<T>(g, pl[cur], lane, jj, gi, gj, k,
-                                               pc[q], pm[q], pp[q]);
-                    p_new[g.idx(g.h + gi, g.h + gj, g.h + k)] = pc[q];
-                    dd_add_prod(hi[0], lo[0], (double)pc[q], (double)y);
-                }
-                pm[q] = pc[q];
-                pc[q] = pp[q];
-            }
-            __syncthreads();
-            cur ^= 1;
-        }
-    }
-    grid_reduce<1>(hi, lo, rb, [&](double* sm) { st->alpha = st->rho / sm[0]; });
+    DirEpiT<T> epi{g, p_new};
+    tile3_march<T, true>(g, tg, r, p_old, first, beta, pl, epi);
+    grid_reduce<1>(epi.hi, epi.lo, rb,
+                   [&](double* sm) { st->alpha = st->rho / sm[0]; });
 }
 
 template <class T>
@@ -421,50 +263,9 @@ __global__ __launch_bounds__(256) void k_pcg_update3(
     if (st->done)
         return;
     __shared__ T pl[2][TY + 2][TX + 2];
-    const T alpha = (T)st->alpha;
-    double hi[2] = {0.0, 0.0}, lo[2] = {0.0, 0.0};
-    const Tile3Pos tp = tile3_pos(g, tg);
-    if (tp.active) {
-        const int lane = threadIdx.x & (TX - 1);
-        const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / TX);
-        const int gi = tp.i0 + lane;
-        auto pv = [&](int a, int b, int c) { return load3<T>(g, p, a, b, c); };
-        T pm[RY], pc[RY], pp[RY];
-#pragma unroll
-        for (int q = 0; q < RY; ++q)
-            pm[q] = pv(gi, tp.j0 + w * RY + q, tp.k0 - 1);
-        tile3_fill<T>(pl[0], tp.i0, tp.j0, tp.k0, pc, pv, pv);
-        __syncthreads();
-        int cur = 0;
-        for (int k = tp.k0; k < tp.k1; ++k) {
-            tile3_fill<T>(pl[cur ^ 1], tp.i0, tp.j0, k + 1, pp, pv, pv);
-#pragma unroll
-            for (int q = 0; q < RY; ++q) {
-                const int jj = w * RY + q;
-                const int gj = tp.j0 + jj;
-                if (gi < g.n[0] && gj < g.n[1]) {
-                    const T qv = tile3_apply<T>(g, pl[cur], lane, jj, gi, gj, k,
-                                                pc[q], pm[q], pp[q]);
-                    const int64_t c = g.idx(g.h + gi, g.h + gj, g.h + k);
-                    x[c] = fma_t<T>(alpha, pc[q], x[c]);
-                    const T rn = fma_t<T>(-alpha, qv, r[c]);
-                    r[c] = rn;
-                    const T z = dinv_pick<T>(g,
-                                             wall_count(g, 1, gj) +
-                                                 wall_count(g, 2, k),
-                                             wall_count(g, 0, gi)) *
-                                rn;
-                    dd_add_prod(hi[0], lo[0], (double)rn, (double)rn);
-                    dd_add_prod(hi[1], lo[1], (double)rn, (double)z);
-                }
-                pm[q] = pc[q];
-                pc[q] = pp[q];
-            }
-            __syncthreads();
-            cur ^= 1;
-        }
-    }
-    grid_reduce<2>(hi, lo, rb, [&](double* sm) {
+    UpdEpiT<T> epi{g, x, r, (T)st->alpha};
+    tile3_march<T, false>(g, tg, p, nullptr, false, (T)0, pl, epi);
+    grid_reduce<2>(epi.hi, epi.lo, rb, [&](double* sm) {
         const int it = st->iter + 1;
         st->iter = it;
         const double res = sqrt(sm[0]) / st->bnorm;
