@@ -145,13 +145,25 @@ def main():
         s32.close()
         del s32
         v32 = ncell * args.steps / t32
+        k32 = {}
+        for k, nm in ((_lib.TK_DIR_SPMV32, "dir_spmv"), (_lib.TK_UPDATE32, "update")):
+            ms, cnt = kt32[k]
+            if cnt:
+                gbs = WORDS[nm] * 4 * ncell / (ms / cnt * 1e-3) / 1e9
+                k32[nm] = {"avg_us": round(1e3 * ms / cnt, 3), "launches": cnt,
+                           "gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        ms, cnt = kt32[_lib.TK_RESIDUAL]
+        if cnt:
+            k32["fp64_residual"] = {"avg_us": round(1e3 * ms / cnt, 3), "launches": cnt}
         out["fp32_refined"] = {
             "value": v32, "unit": "cell-steps/s", "speedup_vs_fp64": round(v32 / value, 3),
             "ms_per_step": 1e3 * t32 / args.steps,
             "inner_iterations_per_step": round(sum(it32) / args.steps, 1),
             "outer_passes_per_step": round(sum(outer32) / args.steps, 2),
             "max_final_residual": max(res32),
+            "refine_mode": "carry" if cfg32["refine_mode"] == K.REFINE_CARRY else "restart",
             "inner_rtol": cfg32["inner_tol"],
+            "kernels": k32,
         }
 
     if not args.no_cpu:

@@ -48,6 +48,7 @@ class CfConfig(C.Structure):
         ("inner_max_iter", C.c_int),
         ("max_outer", C.c_int),
         ("quirks", C.c_int),
+        ("refine_mode", C.c_int),
     ]
 
 

@@ -11,6 +11,7 @@ CELL, FACE_X, FACE_Y, FACE_Z = 0, 1, 2, 3
 SOLID, FREE = 0, 1
 INFLOW_NONE, INFLOW_BOX, INFLOW_SPHERE = 0, 1, 2
 FP64, FP32_REFINED = 0, 1
+REFINE_RESTART, REFINE_CARRY = 0, 1
 QUIRK_RK3_V0 = 1      # TimeIntegrator.hpp:57-58 (3/4 probe uses v0)
 QUIRK_PROJ_BC_U = 2   # VelocityCorrector.hpp:260 (FaceJ BC applied to u)
 
@@ -40,10 +41,11 @@ def base_config(dim=2, n=128):
         cg_tol=1e-8,
         cg_max_iter=20000,
         precision=FP64,
-        inner_tol=1e-4,
+        inner_tol=1e-2,
         inner_max_iter=20000,
         max_outer=50,
         quirks=QUIRK_RK3_V0 | (QUIRK_PROJ_BC_U if dim == 2 else 0),
+        refine_mode=REFINE_CARRY,
     )
 
 
@@ -94,9 +96,12 @@ CONFIG2_SEED = 1
 def random_solve_3d(n=512):
     """Config 3: pressure solve only on an n^3 grid.  Face velocities i.i.d.
     N(0,1) (seed 3), RHS = -div / dx with its mean removed; FP32 inner CG to
-    1e-4 inside FP64 defect correction to 1e-8."""
+    1e-4 inside FP64 defect correction to 1e-8 (each pass restarts the inner
+    CG).  `refine_mode=REFINE_CARRY, inner_tol=1e-2` is the faster variant
+    measured beside it."""
     c = base_config(3, n)
-    c.update(dt=0.002, clamp_dt=0, precision=FP32_REFINED, inner_tol=1e-4)
+    c.update(dt=0.002, clamp_dt=0, precision=FP32_REFINED, inner_tol=1e-4,
+             refine_mode=REFINE_RESTART)
     return c
 
 

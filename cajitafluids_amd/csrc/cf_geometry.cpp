@@ -44,6 +44,14 @@ cf_status build_geometry(const cf_config& c, Geo& g, const char** why)
         *why = "precision must be CF_FP64 or CF_FP32_REFINED";
         return CF_EINVAL;
     }
+    if (c.refine_mode != CF_REFINE_RESTART && c.refine_mode != CF_REFINE_CARRY) {
+        *why = "refine_mode must be CF_REFINE_RESTART or CF_REFINE_CARRY";
+        return CF_EINVAL;
+    }
+    if (!(c.cg_tol > 0) || !(c.inner_tol > 0) || c.max_outer < 0) {
+        *why = "cg_tol and inner_tol must be positive, max_outer >= 0";
+        return CF_EINVAL;
+    }
     g.dim = c.dim;
     g.h = c.halo;
     for (int d = 0; d < 3; ++d) {

@@ -50,8 +50,12 @@ void launch_project(const Geo& g, Fields cur, const double* p, hipStream_t s);
 void launch_pcg_init64(const Geo& g, const double* b, double* x, double* r,
                        PcgState* st, double tol, int maxit, RedBuf rb,
                        hipStream_t s);
+struct RefineState;
+// init32 ends the pass at ||r|| / ||r0|| < max(itol, 0.5 gtol ||b|| / ||r0||)
+// with ||b|| = rs->bnorm.
 void launch_pcg_init32(const Geo& g, const float* r, float* d, PcgState* st,
-                       double tol, int maxit, RedBuf rb, hipStream_t s);
+                       const RefineState* rs, double itol, double gtol,
+                       int maxit, RedBuf rb, hipStream_t s);
 template <class T>
 void launch_pcg_dir_spmv(const Geo& g, const T* r, const T* p_old, T* p_new,
                          PcgState* st, RedBuf rb, hipStream_t s);
@@ -70,10 +74,20 @@ struct RefineState
 };
 void launch_refine_init(const Geo& g, const double* b, double* x64,
                         float* r32, RefineState* rs, RedBuf rb, hipStream_t s);
-void launch_refine_add(const Geo& g, double* x64, const float* d32,
-                       hipStream_t s);
+// x64 += (double) d32; d32 = 0
+void launch_refine_add(const Geo& g, double* x64, float* d32, hipStream_t s);
 void launch_refine_residual(const Geo& g, const double* b, const double* x64,
                             float* r32, RefineState* rs, RedBuf rb,
                             hipStream_t s);
+// Carried-direction mode (cf_pcg.hip): start, and the pass-boundary
+// residual replacement that re-arms the FP32 PCG in `st`.
+void launch_refine_init_carry(const Geo& g, const double* b, double* x64,
+                              float* r32, float* d32, RefineState* rs,
+                              PcgState* st, double itol, double gtol, int maxit,
+                              RedBuf rb, hipStream_t s);
+void launch_refine_residual_carry(const Geo& g, const double* b,
+                                  const double* x64, float* r32,
+                                  RefineState* rs, PcgState* st, double itol,
+                                  double gtol, RedBuf rb, hipStream_t s);
 
 } // namespace cfmi

@@ -73,9 +73,19 @@ __global__ __launch_bounds__(256) void k_pcg_init64(
                    [&](double* s) { init_state(st, s[0], s[1], tol, maxit); });
 }
 
+// Relative FP32 reduction that ends a defect-correction pass started from a
+// residual of norm rref: inner_tol, or less when half the final target
+// (0.5 tol ||b||) is closer, so the last pass stops near the target instead
+// of a full inner_tol past it.
+__device__ __forceinline__ double pass_tol(double itol, double gtol,
+                                           double bnorm, double rref)
+{
+    return fmax(itol, (0.5 * gtol * bnorm) / rref);
+}
+
 __global__ __launch_bounds__(256) void k_pcg_init32(
     Geo g, const float* __restrict__ r, float* __restrict__ d, PcgState* st,
-    double tol, int maxit, RedBuf rb)
+    const RefineState* rs, double itol, double gtol, int maxit, RedBuf rb)
 {
     double hi[2] = {0.0, 0.0}, lo[2] = {0.0, 0.0};
     for_owned_rows(g, [&](int64_t base, int j, int k) {
@@ -89,8 +99,10 @@ __global__ __launch_bounds__(256) void k_pcg_init32(
             dd_add_prod(hi[1], lo[1], (double)rc, (double)rc);
         }
     });
-    grid_reduce<2>(hi, lo, rb,
-                   [&](double* s) { init_state(st, s[0], s[1], tol, maxit); });
+    grid_reduce<2>(hi, lo, rb, [&](double* s) {
+        const double r0 = sqrt(s[1]);
+        init_state(st, s[0], s[1], pass_tol(itol, gtol, rs->bnorm, r0), maxit);
+    });
 }
 
 // ---- one PCG iteration --------------------------------------------------------
@@ -306,10 +318,11 @@ __global__ __launch_bounds__(256) void k_refine_init(
 
 __global__ __launch_bounds__(256) void k_refine_add(Geo g,
                                                      double* __restrict__ x64,
-                                                     const float* __restrict__ d)
+                                                     float* __restrict__ d)
 {
     for_owned_cells(g, [&](int64_t c, int, int, int) {
         x64[c] = x64[c] + (double)d[c];
+        d[c] = 0.0f;
     });
 }
 
@@ -332,6 +345,87 @@ __global__ __launch_bounds__(256) void k_refine_residual(
     });
     grid_reduce<1>(hi, lo, rb,
                    [&](double* s) { rst->res = sqrt(s[0]) / rst->bnorm; });
+}
+
+// ---- defect correction with the search direction carried across passes -----
+//   x = 0; bnorm = ||b||; r32 = (float) b; d = 0; rref = ||r32||;
+//   rho = r32.(D32^-1 r32); then one FP32 PCG run (the iteration kernels
+//   above, which stop when ||r|| / rref < pass_tol).  At each pass boundary:
+//   x = x + (double) d; d = 0; r64 = b - A x; res = ||r64|| / bnorm;
+//   r32 = (float) r64; rref = ||r32||; beta = r32.(D32^-1 r32) / rho;
+//   rho = r32.(D32^-1 r32); the PCG resumes with the same direction.
+__global__ __launch_bounds__(256) void k_refine_init_carry(
+    Geo g, const double* __restrict__ b, double* __restrict__ x64,
+    float* __restrict__ r32, float* __restrict__ d32, RefineState* rst,
+    PcgState* st, double itol, double gtol, int maxit, RedBuf rb)
+{
+    double hi[3] = {0.0, 0.0, 0.0}, lo[3] = {0.0, 0.0, 0.0};
+    for_owned_rows(g, [&](int64_t base, int j, int k) {
+        const int ryz = wall_count(g, 1, j) + wall_count(g, 2, k);
+        for (int i = threadIdx.x; i < g.n[0]; i += kBlock) {
+            const int64_t c = base + i;
+            const double bc = b[c];
+            const float rc = (float)bc;
+            const float z = dinv_pick<float>(g, ryz, wall_count(g, 0, i)) * rc;
+            x64[c] = 0.0;
+            r32[c] = rc;
+            d32[c] = 0.0f;
+            dd_add_prod(hi[0], lo[0], bc, bc);
+            dd_add_prod(hi[1], lo[1], (double)rc, (double)rc);
+            dd_add_prod(hi[2], lo[2], (double)rc, (double)z);
+        }
+    });
+    grid_reduce<3>(hi, lo, rb, [&](double* s) {
+        const double bnorm = sqrt(s[0]);
+        const double rref = sqrt(s[1]);
+        rst->bnorm = bnorm;
+        rst->res = 1.0;
+        st->bnorm = rref;
+        st->tol = pass_tol(itol, gtol, bnorm, rref);
+        st->maxit = maxit;
+        st->iter = 0;
+        st->rho = s[2];
+        st->alpha = 0.0;
+        st->beta = 0.0;
+        st->done = bnorm == 0.0 ? 1 : (maxit <= 0 ? 1 : 0);
+        st->converged = bnorm == 0.0 ? 1 : 0;
+        if (bnorm == 0.0)
+            rst->res = 0.0;
+    });
+}
+
+template <int DIM>
+__global__ __launch_bounds__(256) void k_refine_residual_carry(
+    Geo g, const double* __restrict__ b, const double* __restrict__ x64,
+    float* __restrict__ r32, RefineState* rst, PcgState* st, double itol,
+    double gtol, RedBuf rb)
+{
+    double hi[3] = {0.0, 0.0, 0.0}, lo[3] = {0.0, 0.0, 0.0};
+    for_owned_rows(g, [&](int64_t base, int j, int k) {
+        const RowStencil rs = row_stencil(g, j, k);
+        for (int i = threadIdx.x; i < g.n[0]; i += kBlock) {
+            const int64_t c = base + i;
+            const LaneX lx = lane_x(g, i);
+            const double y = apply_A<DIM, double>(g, rs, lx, x64, c);
+            const double r = b[c] - y;
+            const float rc = (float)r;
+            const float z = dinv_pick<float>(g, rs.cnt, lx.cnt) * rc;
+            r32[c] = rc;
+            dd_add_prod(hi[0], lo[0], r, r);
+            dd_add_prod(hi[1], lo[1], (double)rc, (double)rc);
+            dd_add_prod(hi[2], lo[2], (double)rc, (double)z);
+        }
+    });
+    grid_reduce<3>(hi, lo, rb, [&](double* s) {
+        rst->res = sqrt(s[0]) / rst->bnorm;
+        const double rref = sqrt(s[1]);
+        st->bnorm = rref;
+        st->tol = pass_tol(itol, gtol, rst->bnorm, rref);
+        st->beta = s[2] / st->rho;
+        st->rho = s[2];
+        st->done = 0;
+        st->converged = 0;
+    });
 }
 
 // ---- RHS mean removal --------------------------------------------------------
@@ -377,9 +471,32 @@ void launch_pcg_init64(const Geo& g, const double* b, double* x, double* r,
 }
 
 void launch_pcg_init32(const Geo& g, const float* r, float* d, PcgState* st,
-                       double tol, int maxit, RedBuf rb, hipStream_t s)
+                       const RefineState* rs, double itol, double gtol,
+                       int maxit, RedBuf rb, hipStream_t s)
 {
-    CF_STREAM_LAUNCH(k_pcg_init32, g, r, d, st, tol, maxit, rb);
+    CF_STREAM_LAUNCH(k_pcg_init32, g, r, d, st, rs, itol, gtol, maxit, rb);
+}
+
+void launch_refine_init_carry(const Geo& g, const double* b, double* x64,
+                              float* r32, float* d32, RefineState* rs,
+                              PcgState* st, double itol, double gtol, int maxit,
+                              RedBuf rb, hipStream_t s)
+{
+    CF_STREAM_LAUNCH(k_refine_init_carry, g, b, x64, r32, d32, rs, st, itol,
+                     gtol, maxit, rb);
+}
+
+void launch_refine_residual_carry(const Geo& g, const double* b,
+                                  const double* x64, float* r32,
+                                  RefineState* rs, PcgState* st, double itol,
+                                  double gtol, RedBuf rb, hipStream_t s)
+{
+    if (g.dim == 2)
+        CF_STREAM_LAUNCH(k_refine_residual_carry<2>, g, b, x64, r32, rs, st,
+                         itol, gtol, rb);
+    else
+        CF_STREAM_LAUNCH(k_refine_residual_carry<3>, g, b, x64, r32, rs, st,
+                         itol, gtol, rb);
 }
 
 template <class T>
@@ -428,8 +545,7 @@ void launch_refine_init(const Geo& g, const double* b, double* x64,
     CF_STREAM_LAUNCH(k_refine_init, g, b, x64, r32, rs, rb);
 }
 
-void launch_refine_add(const Geo& g, double* x64, const float* d32,
-                       hipStream_t s)
+void launch_refine_add(const Geo& g, double* x64, float* d32, hipStream_t s)
 {
     CF_STREAM_LAUNCH(k_refine_add, g, x64, d32);
 }

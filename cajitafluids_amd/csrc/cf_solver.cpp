@@ -180,7 +180,7 @@ VelocityCorrector::VelocityCorrector(ProblemManager& pm, const cf_config& cfg,
     _r.alloc(n, s);
     _p[0].alloc(n, s);
     _p[1].alloc(n, s);
-    _red_partials.alloc((int64_t)reduction_blocks_max(_g) * 2 * 2, s);
+    _red_partials.alloc((int64_t)reduction_blocks_max(_g) * 3 * 2, s);
     _red_counter.alloc(1, s);
     _scratch.alloc(8, s);
     void* p = nullptr;
@@ -252,21 +252,25 @@ void VelocityCorrector::applyPressure()
 }
 
 // Queue batches of iterations without waiting; poll the device flag once
-// per batch.  The first batch is sized from the previous solve's count so
-// that few no-op launches follow convergence.  p[0]/p[1] alternate as the
-// old and new direction.
+// per batch.  The first batch is sized from a prediction (the previous
+// solve's, or pass's, count) so that few no-op launches follow the stop.
+// p[cur] holds the current direction; the kernels write the next one into
+// p[cur ^ 1].  Returns the device's iteration counter.
 template <class T>
-int VelocityCorrector::runIterations(T* x, T* r, DevArray<T> (&p)[2],
-                                     int& predict, int tk0)
+int VelocityCorrector::runIterations(T* x, T* r, DevArray<T> (&p)[2], int& cur,
+                                     int predict, int tk0)
 {
     RedBuf rb{_red_partials.ptr, _red_counter.ptr};
     PcgState hs;
+    readState(hs);
+    if (hs.done)
+        return hs.iter;
+    const int it0 = hs.iter;
     int launched = 0;
-    int cur = 0;
     int batch = predict > 4 ? predict - 2 : 16;
     for (;;) {
         for (int c = 0; c < batch; ++c) {
-            const int it = launched + c + 1;
+            const int it = it0 + launched + c + 1;
             int t0 = _timer.begin(_s);
             launch_pcg_dir_spmv<T>(_g, r, p[cur].ptr, p[cur ^ 1].ptr, _st, rb, _s);
             _timer.end(t0, _s, tk0 + 0, it);
@@ -282,7 +286,9 @@ int VelocityCorrector::runIterations(T* x, T* r, DevArray<T> (&p)[2],
             break;
         batch = 4;
     }
-    predict = hs.iter;
+    // The no-op launches after the stop still flipped `cur`: recover the
+    // buffer that holds the last real direction.
+    cur ^= (launched - (hs.iter - it0)) & 1;
     return hs.iter;
 }
 
@@ -291,11 +297,11 @@ int VelocityCorrector::runPcg64(const double* b, double* x, double tol,
 {
     RedBuf rb{_red_partials.ptr, _red_counter.ptr};
     launch_pcg_init64(_g, b, x, _r.ptr, _st, tol, maxit, rb, _s);
-    PcgState hs;
-    readState(hs);
-    if (hs.done)
-        return hs.iter;
-    return runIterations<double>(x, _r.ptr, _p, _predict64, TK_DIR_SPMV);
+    int cur = 0;
+    const int its =
+        runIterations<double>(x, _r.ptr, _p, cur, _predict64, TK_DIR_SPMV);
+    _predict64 = its;
+    return its;
 }
 
 cf_status VelocityCorrector::solve64(const double* b, double* x,
@@ -313,61 +319,114 @@ cf_status VelocityCorrector::solve64(const double* b, double* x,
     return hs.converged ? CF_OK : CF_ENOCONV;
 }
 
-int VelocityCorrector::runPcg32(double tol, int maxit)
+void VelocityCorrector::readRefine()
+{
+    CF_HIP(hipMemcpyAsync(_rs_host, _rs, sizeof(RefineState),
+                          hipMemcpyDeviceToHost, _s));
+    CF_HIP(hipStreamSynchronize(_s));
+}
+
+int& VelocityCorrector::passPrediction(int pass)
+{
+    if ((int)_predict_pass.size() <= pass)
+        _predict_pass.resize(pass + 1, 0);
+    return _predict_pass[pass];
+}
+
+// Restart mode: every pass runs a fresh FP32 PCG on the FP32 residual.
+cf_status VelocityCorrector::solveRestart(const double* b, double* x,
+                                          cf_solve_stats& loc)
 {
     RedBuf rb{_red_partials.ptr, _red_counter.ptr};
-    launch_pcg_init32(_g, _r32.ptr, _d32.ptr, _st, tol, maxit, rb, _s);
-    PcgState hs;
-    readState(hs);
-    if (hs.done)
-        return hs.iter;
-    return runIterations<float>(_d32.ptr, _r32.ptr, _p32, _predict32,
-                                TK_DIR_SPMV32);
+    launch_refine_init(_g, b, x, _r32.ptr, _rs, rb, _s);
+    readRefine();
+    loc.b_norm = _rs_host->bnorm;
+    if (_rs_host->bnorm == 0.0) {
+        loc.converged = 1;
+        loc.final_residual = 0.0;
+        return CF_OK;
+    }
+    const int inner_max =
+        _cfg.inner_max_iter > 0 ? _cfg.inner_max_iter : _cfg.cg_max_iter;
+    for (;;) {
+        loc.final_residual = _rs_host->res;
+        if (_rs_host->res < _cfg.cg_tol) {
+            loc.converged = 1;
+            return CF_OK;
+        }
+        if (loc.outer_iterations >= _cfg.max_outer ||
+            loc.iterations >= _cfg.cg_max_iter)
+            return CF_ENOCONV;
+        launch_pcg_init32(_g, _r32.ptr, _d32.ptr, _st, _rs, _cfg.inner_tol,
+                          _cfg.cg_tol, inner_max, rb, _s);
+        int cur = 0;
+        int& pred = passPrediction(loc.outer_iterations);
+        const int its =
+            runIterations<float>(_d32.ptr, _r32.ptr, _p32, cur, pred, TK_DIR_SPMV32);
+        pred = its;
+        loc.iterations += its;
+        launch_refine_add(_g, x, _d32.ptr, _s);
+        int t = _timer.begin(_s);
+        launch_refine_residual(_g, b, x, _r32.ptr, _rs, rb, _s);
+        _timer.end(t, _s, TK_RESIDUAL, 0);
+        readRefine();
+        _timer.resolve(INT_MAX);
+        loc.outer_iterations += 1;
+    }
+}
+
+// Carry mode: one FP32 PCG run whose residual is replaced by the FP64 one at
+// each pass boundary, keeping the search direction.
+cf_status VelocityCorrector::solveCarry(const double* b, double* x,
+                                        cf_solve_stats& loc)
+{
+    RedBuf rb{_red_partials.ptr, _red_counter.ptr};
+    launch_refine_init_carry(_g, b, x, _r32.ptr, _d32.ptr, _rs, _st,
+                             _cfg.inner_tol, _cfg.cg_tol, _cfg.cg_max_iter, rb, _s);
+    readRefine();
+    loc.b_norm = _rs_host->bnorm;
+    loc.final_residual = _rs_host->res;
+    if (_rs_host->bnorm == 0.0) {
+        loc.converged = 1;
+        return CF_OK;
+    }
+    int cur = 0;
+    int done_its = 0;
+    for (;;) {
+        int& pred = passPrediction(loc.outer_iterations);
+        const int its =
+            runIterations<float>(_d32.ptr, _r32.ptr, _p32, cur, pred, TK_DIR_SPMV32);
+        pred = its - done_its;
+        done_its = its;
+        loc.iterations = its;
+        if (!_st_host->converged) // iteration cap reached inside the pass
+            return CF_ENOCONV;
+        launch_refine_add(_g, x, _d32.ptr, _s);
+        int t = _timer.begin(_s);
+        launch_refine_residual_carry(_g, b, x, _r32.ptr, _rs, _st, _cfg.inner_tol,
+                                     _cfg.cg_tol, rb, _s);
+        _timer.end(t, _s, TK_RESIDUAL, 0);
+        readRefine();
+        _timer.resolve(INT_MAX);
+        loc.outer_iterations += 1;
+        loc.final_residual = _rs_host->res;
+        if (_rs_host->res < _cfg.cg_tol) {
+            loc.converged = 1;
+            return CF_OK;
+        }
+        if (loc.outer_iterations >= _cfg.max_outer)
+            return CF_ENOCONV;
+    }
 }
 
 cf_status VelocityCorrector::solveRefined(const double* b, double* x,
                                           cf_solve_stats* st)
 {
     ensure32();
-    RedBuf rb{_red_partials.ptr, _red_counter.ptr};
-    launch_refine_init(_g, b, x, _r32.ptr, _rs, rb, _s);
-    auto read_rs = [&]() {
-        CF_HIP(hipMemcpyAsync(_rs_host, _rs, sizeof(RefineState),
-                              hipMemcpyDeviceToHost, _s));
-        CF_HIP(hipStreamSynchronize(_s));
-    };
-    read_rs();
     cf_solve_stats loc;
     std::memset(&loc, 0, sizeof(loc));
-    loc.b_norm = _rs_host->bnorm;
-    cf_status rc = CF_OK;
-    if (_rs_host->bnorm == 0.0) {
-        loc.converged = 1;
-        loc.final_residual = 0.0;
-    } else {
-        const int inner_max =
-            _cfg.inner_max_iter > 0 ? _cfg.inner_max_iter : _cfg.cg_max_iter;
-        for (;;) {
-            loc.final_residual = _rs_host->res;
-            if (_rs_host->res < _cfg.cg_tol) {
-                loc.converged = 1;
-                break;
-            }
-            if (loc.outer_iterations >= _cfg.max_outer ||
-                loc.iterations >= _cfg.cg_max_iter) {
-                rc = CF_ENOCONV;
-                break;
-            }
-            loc.iterations += runPcg32(_cfg.inner_tol, inner_max);
-            launch_refine_add(_g, x, _d32.ptr, _s);
-            int t = _timer.begin(_s);
-            launch_refine_residual(_g, b, x, _r32.ptr, _rs, rb, _s);
-            _timer.end(t, _s, TK_RESIDUAL, 0);
-            read_rs();
-            _timer.resolve(INT_MAX);
-            loc.outer_iterations += 1;
-        }
-    }
+    cf_status rc = _cfg.refine_mode == CF_REFINE_CARRY ? solveCarry(b, x, loc)
+                                                       : solveRestart(b, x, loc);
     if (st)
         *st = loc;
     return rc;

@@ -150,11 +150,15 @@ class VelocityCorrector
   private:
     cf_status solve64(const double* b, double* x, cf_solve_stats* st);
     cf_status solveRefined(const double* b, double* x, cf_solve_stats* st);
+    cf_status solveRestart(const double* b, double* x, cf_solve_stats& loc);
+    cf_status solveCarry(const double* b, double* x, cf_solve_stats& loc);
     template <class T>
-    int runIterations(T* x, T* r, DevArray<T> (&p)[2], int& predict, int tk0);
+    int runIterations(T* x, T* r, DevArray<T> (&p)[2], int& cur, int predict,
+                      int tk0);
     int runPcg64(const double* b, double* x, double tol, int maxit);
-    int runPcg32(double tol, int maxit);
     void readState(PcgState& host);
+    void readRefine();
+    int& passPrediction(int pass);
     void ensure32();
 
     ProblemManager& _pm;
@@ -170,7 +174,8 @@ class VelocityCorrector
     RefineState* _rs = nullptr;    // device
     PcgState* _st_host = nullptr;  // pinned
     RefineState* _rs_host = nullptr;
-    int _predict64 = 0, _predict32 = 0;
+    int _predict64 = 0;
+    std::vector<int> _predict_pass; // FP32 iterations of each refinement pass
 };
 
 // Solver: src/Solver.hpp:54-279 (setup / step / _addInputs; the dt clamp is

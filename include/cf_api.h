@@ -36,6 +36,7 @@ enum { CF_CURRENT = 0, CF_NEXT = 1 };
 enum { CF_SOLID = 0, CF_FREE = 1 }; /* BoundaryType (BoundaryConditions.hpp:22-29) */
 enum { CF_INFLOW_NONE = 0, CF_INFLOW_BOX = 1, CF_INFLOW_SPHERE = 2 };
 enum { CF_FP64 = 0, CF_FP32_REFINED = 1 };
+enum { CF_REFINE_RESTART = 0, CF_REFINE_CARRY = 1 };
 
 /* As-written reference behaviours, reproduced when set. */
 #define CF_QUIRK_RK3_V0 1    /* 3/4-step probe uses v0: TimeIntegrator.hpp:57-58 */
@@ -78,6 +79,10 @@ typedef struct cf_config {
     int inner_max_iter;
     int max_outer;        /* defect-correction passes */
     int quirks;           /* CF_QUIRK_* */
+    int refine_mode;      /* CF_REFINE_RESTART: each pass restarts the FP32
+                             PCG; CF_REFINE_CARRY: passes keep the search
+                             direction (one FP32 PCG run whose residual is
+                             replaced by the FP64 one at each pass) */
 } cf_config;
 
 typedef struct cf_solve_stats {

@@ -816,7 +816,17 @@ static void cfo_fill_ghosts_f32(struct cfo_state* s, float* a)
     }
 }
 
-static int cfo_pcg32_inner(struct cfo_state* s, double tol, int maxit)
+/* Relative FP32 reduction that ends a defect-correction pass started from a
+ * residual of norm rref: inner_tol, or less when half the final target
+ * (0.5 tol ||b||) is already closer, so the last pass stops near the target
+ * instead of a full inner_tol past it. */
+static double cfo_pass_tol(const cfo_config* c0, double bnorm, double rref)
+{
+    return fmax(c0->inner_tol, (0.5 * c0->cg_tol * bnorm) / rref);
+}
+
+static int cfo_pcg32_inner(struct cfo_state* s, const cfo_config* c0,
+                           double bnorm, int maxit)
 {
     float *d = s->x32, *r = s->r32, *z = s->z32, *p = s->p32, *q = s->q32;
     const double* M = s->M;
@@ -827,6 +837,7 @@ static int cfo_pcg32_inner(struct cfo_state* s, double tol, int maxit)
     double r0 = sqrt(cfo_dot32(s, r, r));
     if (r0 == 0.0)
         return 0;
+    double tol = cfo_pass_tol(c0, bnorm, r0);
     int k = 0;
     while (k < maxit) {
         ++k;
@@ -887,7 +898,7 @@ static int cfo_pcg_refined(struct cfo_state* s, const double* b, double* x,
         }
         float* r32 = s->r32;
         CFO_FOR_OWNED_CELLS(s, r32[c] = (float)r64[c]);
-        st->iterations += cfo_pcg32_inner(s, c0->inner_tol, inner_max);
+        st->iterations += cfo_pcg32_inner(s, c0, bnorm, inner_max);
         float* d = s->x32;
         CFO_FOR_OWNED_CELLS(s, x[c] = x[c] + (double)d[c]);
         if (periodic)
@@ -895,6 +906,95 @@ static int cfo_pcg_refined(struct cfo_state* s, const double* b, double* x,
         cfo_spmv(s, x, y);
         CFO_FOR_OWNED_CELLS(s, r64[c] = b[c] - y[c]);
         st->outer_iterations += 1;
+    }
+}
+
+/*
+ * Defect correction with the search direction carried across passes (a
+ * reliable-update form of the same FP32-inner / FP64-outer scheme):
+ *   x = 0; bnorm = ||b||; r32 = (float) b; d = 0; rref = ||r32||
+ *   rho = r32.(M32 r32); one FP32 Jacobi-PCG run, iteration k:
+ *     p = M32 r (k = 1) or fmaf(beta32, p, M32 r); q = A32 p
+ *     alpha = rho / p.q; d = fmaf(a32, p, d); r = fmaf(-a32, q, r)
+ *     if ||r|| / rref < inner_tol:                   (pass boundary)
+ *        x = x + (double) d; d = 0; r64 = b - A x; res = ||r64|| / bnorm
+ *        stop if res < tol (or max_outer passes)
+ *        r = (float) r64; rref = ||r||; rz = r.(M32 r)
+ *     else rz = r.(M32 r)
+ *     beta = rz / rho; rho = rz
+ */
+static int cfo_pcg_carry(struct cfo_state* s, const double* b, double* x,
+                         cfo_stats* st)
+{
+    const cfo_config* c0 = &s->c;
+    int periodic = c0->periodic[0] || c0->periodic[1] || c0->periodic[2];
+    float *d = s->x32, *r = s->r32, *z = s->z32, *p = s->p32, *q = s->q32;
+    const double* M = s->M;
+    double* r64 = s->r;
+    double* y = s->q;
+    double bnorm = sqrt(cfo_dot_dd(s, b, b));
+    st->b_norm = bnorm;
+    st->iterations = 0;
+    st->outer_iterations = 0;
+    CFO_FOR_OWNED_CELLS(s, x[c] = 0.0);
+    if (bnorm == 0.0) {
+        st->converged = 1;
+        st->final_residual = 0.0;
+        return CFO_OK;
+    }
+    st->final_residual = 1.0;
+    CFO_FOR_OWNED_CELLS(s, r[c] = (float)b[c]; d[c] = 0.0f;
+                        z[c] = (float)M[c] * r[c]);
+    double rref = sqrt(cfo_dot32(s, r, r));
+    double rho = cfo_dot32(s, r, z);
+    double beta = 0.0;
+    int k = 0;
+    for (;;) {
+        if (k >= c0->cg_max_iter) {
+            st->converged = 0;
+            st->iterations = k;
+            return CFO_ENOCONV;
+        }
+        ++k;
+        if (k == 1) {
+            CFO_FOR_OWNED_CELLS(s, p[c] = z[c]);
+        } else {
+            float b32 = (float)beta;
+            CFO_FOR_OWNED_CELLS(s, p[c] = fmaf(b32, p[c], z[c]));
+        }
+        if (periodic)
+            cfo_fill_ghosts_f32(s, p);
+        cfo_spmv32(s, p, q);
+        double alpha = rho / cfo_dot32(s, p, q);
+        float a32 = (float)alpha;
+        CFO_FOR_OWNED_CELLS(s, d[c] = fmaf(a32, p[c], d[c]);
+                            r[c] = fmaf(-a32, q[c], r[c]));
+        double rr = cfo_dot32(s, r, r);
+        if (sqrt(rr) / rref < cfo_pass_tol(c0, bnorm, rref)) {
+            CFO_FOR_OWNED_CELLS(s, x[c] = x[c] + (double)d[c]; d[c] = 0.0f);
+            if (periodic)
+                cfo_fill_ghosts_array(s, 0, x);
+            cfo_spmv(s, x, y);
+            CFO_FOR_OWNED_CELLS(s, r64[c] = b[c] - y[c]);
+            double res = sqrt(cfo_dot_dd(s, r64, r64)) / bnorm;
+            st->outer_iterations += 1;
+            st->final_residual = res;
+            st->iterations = k;
+            if (res < c0->cg_tol) {
+                st->converged = 1;
+                return CFO_OK;
+            }
+            if (st->outer_iterations >= c0->max_outer) {
+                st->converged = 0;
+                return CFO_ENOCONV;
+            }
+            CFO_FOR_OWNED_CELLS(s, r[c] = (float)r64[c]);
+            rref = sqrt(cfo_dot32(s, r, r));
+        }
+        CFO_FOR_OWNED_CELLS(s, z[c] = (float)M[c] * r[c]);
+        double rz = cfo_dot32(s, r, z);
+        beta = rz / rho;
+        rho = rz;
     }
 }
 
@@ -917,6 +1017,8 @@ int cfo_pressure_solve(cfo_state* s, int precision, cfo_stats* st)
             if (!s->x32 || !s->r32 || !s->z32 || !s->p32 || !s->q32)
                 return CFO_ENOMEM;
         }
+        if (s->c.refine_mode == CFO_REFINE_CARRY)
+            return cfo_pcg_carry(s, s->rhs, s->lhs, st);
         return cfo_pcg_refined(s, s->rhs, s->lhs, st);
     }
     return cfo_pcg64(s, s->rhs, s->lhs, s->c.cg_tol, s->c.cg_max_iter, st);

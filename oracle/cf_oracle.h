@@ -84,7 +84,10 @@ typedef struct cfo_config {
     int inner_max_iter;
     int max_outer;
     int quirks;
+    int refine_mode; /* CFO_REFINE_RESTART or CFO_REFINE_CARRY */
 } cfo_config;
+
+enum { CFO_REFINE_RESTART = 0, CFO_REFINE_CARRY = 1 };
 
 typedef struct cfo_stats {
     int converged;

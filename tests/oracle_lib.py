@@ -19,6 +19,7 @@ CURRENT, NEXT = 0, 1
 SOLID, FREE = 0, 1
 INFLOW_NONE, INFLOW_BOX, INFLOW_SPHERE = 0, 1, 2
 FP64, FP32_REFINED = 0, 1
+REFINE_RESTART, REFINE_CARRY = 0, 1
 QUIRK_RK3_V0, QUIRK_PROJ_BC_U = 1, 2
 
 
@@ -51,6 +52,7 @@ class CfoConfig(C.Structure):
         ("inner_max_iter", C.c_int),
         ("max_outer", C.c_int),
         ("quirks", C.c_int),
+        ("refine_mode", C.c_int),
     ]
 
 

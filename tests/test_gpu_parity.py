@@ -58,10 +58,13 @@ def test_pcg_fp64_random_rhs(cf, oracle_mod, dim, n):
     assert st_g["final_residual"] < cfg["cg_tol"]
 
 
-@pytest.mark.parametrize("dim,n", [(2, 32), (3, 12)])
-def test_pcg_fp32_refined_random_rhs(cf, oracle_mod, dim, n):
+@pytest.mark.parametrize("mode,itol", [(K.REFINE_CARRY, 1e-2), (K.REFINE_RESTART, 1e-4),
+                                       (K.REFINE_CARRY, 1e-4)])
+@pytest.mark.parametrize("dim,n", [(2, 32), (3, 12), (3, 40)])
+def test_pcg_fp32_refined_random_rhs(cf, oracle_mod, dim, n, mode, itol):
     cfg = K.base_config(dim, n)
-    cfg.update(clamp_dt=0, dt=0.002, precision=K.FP32_REFINED)
+    cfg.update(clamp_dt=0, dt=0.002, precision=K.FP32_REFINED, refine_mode=mode,
+               inner_tol=itol)
     sol, orc = _pair(cf, oracle_mod, cfg)
     sol.init_random_faces(3)
     gpu_to_oracle(sol, orc)
