@@ -35,10 +35,30 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-fp64", action="store_true",
+                    help="skip the FP64 leg (profiling runs only; no JSON line)")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--timing-sample", type=int, default=8,
+                    help="time the PCG kernels with HIP events in one "
+                         "iteration out of N inside the timed region (0: off)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first run a plain 1 GiB device copy, the reference "
+                         "kernel for scaling rocprof FETCH_SIZE / WRITE_SIZE")
     return ap.parse_args()
+
+
+def copy_probe():
+    """One plain device-to-device copy of 2^27 doubles (1 GiB read, 1 GiB
+    written), so that a counter run over this command also holds the
+    counter values of a copy of known size."""
+    import torch
+    a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
 
 
 def make_solver(cfg):
@@ -80,11 +100,20 @@ def main():
         sys.exit(2)
     from cajitafluids_amd import _lib, configs as K
 
+    if args.copy_probe:
+        copy_probe()
     n = args.n
     ncell = n ** 3
+    if args.no_fp64:
+        cfg32 = K.vortex_tank_3d(n, precision=K.FP32_REFINED)
+        s32 = make_solver(cfg32)
+        timed_run(s32, args.steps, args.warmup)
+        s32.close()
+        return
     cfg = K.vortex_tank_3d(n)
     sol = make_solver(cfg)
-    t64, it64, _, res64, kt = timed_run(sol, args.steps, args.warmup)
+    t64, it64, _, res64, kt = timed_run(sol, args.steps, args.warmup,
+                                        timing=args.timing_sample)
     dt_eff = sol.dt
     sol.close()
     del sol
@@ -98,16 +127,21 @@ def main():
             per[nm] = dict(avg_us=1e3 * ms / cnt, launches=cnt, total_ms=ms,
                            bytes=WORDS[nm] * 8 * ncell)
     fin_ms, fin_n = kt[_lib.TK_FINALIZE]
-    t_pcg = (sum(p["total_ms"] for p in per.values()) + fin_ms) / 1e3
+    # Kernel time per iteration from the sampled launches, times the
+    # iterations of the timed region.
+    t_pcg = sum(p["avg_us"] for p in per.values()) * 1e-6 * sum(it64)
     total_iters = sum(it64)
     pcg_gbs = 88.0 * ncell * total_iters / t_pcg / 1e9 if t_pcg > 0 else None
-    dom = max(per, key=lambda k: per[k]["total_ms"])
-    d = per[dom]
-    achieved = d["bytes"] / (d["avg_us"] * 1e-6) / 1e9
-    roofline = {"bound": "hbm", "kernel": f"k_pcg_{dom}", "achieved": round(achieved, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None,
-                "bytes_per_launch": d["bytes"], "avg_us": round(d["avg_us"], 2)}
+    roofline = None
+    if per:
+        # dominant kernel = the one with the most device time per iteration
+        dom = max(per, key=lambda k: per[k]["avg_us"])
+        d = per[dom]
+        achieved = d["bytes"] / (d["avg_us"] * 1e-6) / 1e9
+        roofline = {"bound": "hbm", "kernel": f"k_pcg_{dom}", "achieved": round(achieved, 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "bytes_per_launch": d["bytes"], "avg_us": round(d["avg_us"], 2)}
     adv_ms, adv_n = kt[_lib.TK_ADVECT]
 
     out = {
@@ -141,7 +175,8 @@ def main():
     if not args.no_fp32:
         cfg32 = K.vortex_tank_3d(n, precision=K.FP32_REFINED)
         s32 = make_solver(cfg32)
-        t32, it32, outer32, res32, kt32 = timed_run(s32, args.steps, args.warmup)
+        t32, it32, outer32, res32, kt32 = timed_run(s32, args.steps, args.warmup,
+                                                    timing=args.timing_sample)
         s32.close()
         del s32
         v32 = ncell * args.steps / t32

@@ -369,6 +369,7 @@ cf_status cf_set_kernel_timing(cf_solver* h, int enable)
 {
     return guarded(h, [&](cfmi::Solver& s) {
         s.timer().enabled = enable != 0;
+        s.timer().sample = enable > 1 ? enable : 1;
         return CF_OK;
     });
 }

@@ -37,6 +37,35 @@ __device__ __forceinline__ void dd_add_prod(double& hi, double& lo, double a,
     dd_add(hi, lo, p, pe);
 }
 
+// Per-lane accumulation of a product into an unnormalised pair: hi takes the
+// rounded running sum, and the exact error of every addition (TwoSum) and of
+// the product (TwoProd) is summed into lo.  lo is only ever a small multiple
+// of ulp(hi), so its own rounding is ~2^-100 of the total; dd_norm
+// renormalises before the pair enters dd_add.  EXACT: the product of the two
+// operands is known to be exact in double (both are floats), so the TwoProd
+// error term is skipped.
+template <bool EXACT = false>
+__device__ __forceinline__ void dd_acc_prod(double& hi, double& lo, double a,
+                                            double b)
+{
+    const double p = a * b;
+    const double s = hi + p;
+    const double t = s - hi;
+    double e = (hi - (s - t)) + (p - t);
+    if constexpr (!EXACT)
+        e = e + fma(a, b, -p);
+    lo = lo + e;
+    hi = s;
+}
+
+__device__ __forceinline__ void dd_norm(double& hi, double& lo)
+{
+    const double s = hi + lo;
+    const double t = s - hi;
+    lo = (hi - (s - t)) + (lo - t);
+    hi = s;
+}
+
 // ---- B-spline sampling (Cajita SplineData / evaluateSpline / G2P::value,
 // called at src/Interpolation.hpp:30-41) -------------------------------------
 template <int ORDER>

@@ -191,18 +191,26 @@ __global__ __launch_bounds__(256) void k_pcg_update(
 }
 
 // ---- 3-D kernels (cf_tile3.hpp) ------------------------------------------------
+// Products of two floats are exact in double, so the FP32 kernels skip the
+// TwoProd error term.
+template <class T>
+constexpr bool exact_products()
+{
+    return std::is_same<T, float>::value;
+}
+
 template <class T>
 struct DirEpiT
 {
     const Geo& g;
     T* __restrict__ p_new;
     double hi[1] = {0.0}, lo[1] = {0.0};
-    __device__ void prefetch(const Tile3Ctx&, int) {}
+    __device__ void prefetch(const Tile3Ctx&, const Plane3Meta&, int) {}
     __device__ void rotate() {}
-    __device__ void cell(const Tile3Ctx& c, int q, int, int, int k, T pc, T y)
+    __device__ void cell(const Tile3Ctx& c, const Plane3Meta& m, int q, T pc, T y)
     {
-        p_new[c.colb[q] + g.px * g.py * (int64_t)(g.h + k)] = pc;
-        dd_add_prod(hi[0], lo[0], (double)pc, (double)y);
+        p_new[c.colb[q] + m.off] = pc;
+        dd_acc_prod<exact_products<T>()>(hi[0], lo[0], (double)pc, (double)y);
     }
 };
 
@@ -215,13 +223,12 @@ struct UpdEpiT
     T alpha;
     double hi[2] = {0.0, 0.0}, lo[2] = {0.0, 0.0};
     T xc[tile3::RY], rc[tile3::RY], xn[tile3::RY], rn_[tile3::RY];
-    __device__ void prefetch(const Tile3Ctx& c, int k)
+    __device__ void prefetch(const Tile3Ctx& c, const Plane3Meta& m, int k)
     {
 #pragma unroll
         for (int q = 0; q < tile3::RY; ++q) {
-            const int gj = c.j0 + c.w * tile3::RY + q;
-            const bool ok = k < c.k1 && c.i0 + c.lane < g.n[0] && gj < g.n[1];
-            const int64_t at = c.colb[q] + g.px * g.py * (int64_t)(g.h + k);
+            const bool ok = k < c.k1 && c.valid[q];
+            const int64_t at = c.colb[q] + m.off;
             xn[q] = ok ? x[at] : (T)0;
             rn_[q] = ok ? r[at] : (T)0;
         }
@@ -234,18 +241,15 @@ struct UpdEpiT
             rc[q] = rn_[q];
         }
     }
-    __device__ void cell(const Tile3Ctx& c, int q, int gi, int gj, int k, T pc,
-                         T qv)
+    __device__ void cell(const Tile3Ctx& c, const Plane3Meta& m, int q, T pc, T qv)
     {
-        const int64_t at = c.colb[q] + g.px * g.py * (int64_t)(g.h + k);
+        const int64_t at = c.colb[q] + m.off;
         x[at] = fma_t<T>(alpha, pc, xc[q]);
         const T rn = fma_t<T>(-alpha, qv, rc[q]);
         r[at] = rn;
-        const T z = dinv_pick<T>(g, wall_count(g, 1, gj) + wall_count(g, 2, k),
-                                 wall_count(g, 0, gi)) *
-                    rn;
-        dd_add_prod(hi[0], lo[0], (double)rn, (double)rn);
-        dd_add_prod(hi[1], lo[1], (double)rn, (double)z);
+        const T z = dinv_pick<T>(g, c.wy[q] + m.wz, c.wx) * rn;
+        dd_acc_prod<exact_products<T>()>(hi[0], lo[0], (double)rn, (double)rn);
+        dd_acc_prod<exact_products<T>()>(hi[1], lo[1], (double)rn, (double)z);
     }
 };
 
@@ -262,6 +266,7 @@ __global__ __launch_bounds__(256) void k_pcg_dir_spmv3(
     const T beta = (T)st->beta;
     DirEpiT<T> epi{g, p_new};
     tile3_march<T, true>(g, tg, r, p_old, first, beta, pl, epi);
+    dd_norm(epi.hi[0], epi.lo[0]);
     grid_reduce<1>(epi.hi, epi.lo, rb,
                    [&](double* sm) { st->alpha = st->rho / sm[0]; });
 }
@@ -277,6 +282,8 @@ __global__ __launch_bounds__(256) void k_pcg_update3(
     __shared__ T pl[2][TY + 2][TX + 2];
     UpdEpiT<T> epi{g, x, r, (T)st->alpha};
     tile3_march<T, false>(g, tg, p, nullptr, false, (T)0, pl, epi);
+    dd_norm(epi.hi[0], epi.lo[0]);
+    dd_norm(epi.hi[1], epi.lo[1]);
     grid_reduce<2>(epi.hi, epi.lo, rb, [&](double* sm) {
         const int it = st->iter + 1;
         st->iter = it;

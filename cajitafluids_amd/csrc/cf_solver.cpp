@@ -271,10 +271,10 @@ int VelocityCorrector::runIterations(T* x, T* r, DevArray<T> (&p)[2], int& cur,
     for (;;) {
         for (int c = 0; c < batch; ++c) {
             const int it = it0 + launched + c + 1;
-            int t0 = _timer.begin(_s);
+            int t0 = _timer.begin_iter(_s, it);
             launch_pcg_dir_spmv<T>(_g, r, p[cur].ptr, p[cur ^ 1].ptr, _st, rb, _s);
             _timer.end(t0, _s, tk0 + 0, it);
-            int t2 = _timer.begin(_s);
+            int t2 = _timer.begin_iter(_s, it);
             launch_pcg_update<T>(_g, x, r, p[cur ^ 1].ptr, _st, rb, _s);
             _timer.end(t2, _s, tk0 + 1, it);
             cur ^= 1;

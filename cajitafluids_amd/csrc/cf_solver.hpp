@@ -70,8 +70,14 @@ class KernelTimer
   public:
     ~KernelTimer();
     bool enabled = false;
+    int sample = 1; // PCG kernels: time one iteration in `sample`
     // Record a start event; returns a slot for end().
     int begin(hipStream_t s);
+    // begin() for a kernel of PCG iteration `it` (sampled).
+    int begin_iter(hipStream_t s, int it)
+    {
+        return (sample <= 1 || it % sample == 0) ? begin(s) : -1;
+    }
     void end(int slot, hipStream_t s, int kernel, int iter);
     // Resolve pending pairs (stream must be synchronised); pairs whose tag is
     // not below `limit` were no-op launches after convergence and are dropped.

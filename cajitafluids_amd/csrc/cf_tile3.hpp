@@ -10,6 +10,11 @@
 // memory latency.  Every vector element is read once per chunk; halo cells
 // come from the neighbouring tiles, which the XCD-ordered block mapping keeps
 // in the same L2.
+//
+// Everything that depends on the lane's x position is computed once per
+// block, everything that depends on a row once per block and wave (rows are
+// wave-uniform), and everything that depends on the plane once per plane, so
+// the per-cell work is the arithmetic itself.
 #pragma once
 
 #include "cf_stream.hpp"
@@ -74,19 +79,35 @@ __device__ __forceinline__ T dinv_sel(const Geo& g, int cnt)
     return v;
 }
 
+// Whether the -/+ coupling of an owned cell with global index gi along axis
+// d is cut by a wall (BoundaryConditions.hpp:57-97: gi <= min, gi > max - 1).
+__device__ __forceinline__ bool wall_lo(const Geo& g, int d, int gi)
+{
+    return !g.periodic[d] && gi <= 0;
+}
+__device__ __forceinline__ bool wall_hi(const Geo& g, int d, int gi)
+{
+    return !g.periodic[d] && gi > g.n[d] - 2;
+}
+
 // The block's place in the grid and its lanes' load slots.  Slot q < RY is
 // the lane's own cell in row w*RY + q; slot RY is the lane's halo cell (if
 // any).  colb is the linear index of the slot's (x, y) column at local
-// z = 0, or -1 for a zero ghost; wx / wxy are its x and x+y wall counts.
+// z = 0, or -1 for a zero ghost.
 struct Tile3Ctx
 {
     bool active;
     int i0, j0, k0, k1;
     int lane, w;
     int64_t colb[tile3::NS];
-    int wxy[tile3::NS];
-    int wx;          // own cells' x wall count
-    int hrow, hcol;  // LDS position of the halo slot (hrow < 0: none)
+    int hwxy;           // x + y wall count of the halo slot
+    int hrow, hcol;     // LDS position of the halo slot (hrow < 0: none)
+    // own cells
+    int wx;             // x wall count of the lane's column
+    bool xwall_m, xwall_p;
+    int wy[tile3::RY];  // y wall count of each row (wave-uniform)
+    bool ywall_m[tile3::RY], ywall_p[tile3::RY];
+    bool valid[tile3::RY];
 };
 
 __device__ __forceinline__ Tile3Ctx tile3_ctx(const Geo& g, const Tile3Grid& t)
@@ -105,21 +126,27 @@ __device__ __forceinline__ Tile3Ctx tile3_ctx(const Geo& g, const Tile3Grid& t)
     c.k1 = min(c.k0 + t.kz, g.n[2]);
     c.lane = threadIdx.x & (TX - 1);
     c.w = __builtin_amdgcn_readfirstlane(threadIdx.x / TX);
-    auto col = [&](int gi, int gj, int& wxy) -> int64_t {
+    auto col = [&](int gi, int gj) -> int64_t {
         const int a = wrap_or_ghost(g, 0, gi), b = wrap_or_ghost(g, 1, gj);
-        if (a < 0 || b < 0) {
-            wxy = 0;
+        if (a < 0 || b < 0)
             return -1;
-        }
-        wxy = wall_count(g, 0, a) + wall_count(g, 1, b);
         return g.sx + (g.h + a) + g.px * (int64_t)(g.h + b);
     };
     const int gi = c.i0 + c.lane;
     const int a = wrap_or_ghost(g, 0, gi);
     c.wx = a < 0 ? 0 : wall_count(g, 0, a);
+    c.xwall_m = wall_lo(g, 0, gi);
+    c.xwall_p = wall_hi(g, 0, gi);
 #pragma unroll
-    for (int q = 0; q < RY; ++q)
-        c.colb[q] = col(gi, c.j0 + c.w * RY + q, c.wxy[q]);
+    for (int q = 0; q < RY; ++q) {
+        const int gj = c.j0 + c.w * RY + q;
+        const int b = wrap_or_ghost(g, 1, gj);
+        c.colb[q] = col(gi, gj);
+        c.wy[q] = b < 0 ? 0 : wall_count(g, 1, b);
+        c.ywall_m[q] = wall_lo(g, 1, gj);
+        c.ywall_p[q] = wall_hi(g, 1, gj);
+        c.valid[q] = gi < g.n[0] && gj < g.n[1];
+    }
     const int hh = threadIdx.x;
     int gih = 0, gjh = 0;
     c.hrow = -1;
@@ -134,20 +161,32 @@ __device__ __forceinline__ Tile3Ctx tile3_ctx(const Geo& g, const Tile3Grid& t)
         c.hrow = hh - 2 * TX - TY + 1; c.hcol = TX + 1; gih = c.i0 + TX;
         gjh = c.j0 + hh - 2 * TX - TY;
     }
-    c.colb[RY] = c.hrow >= 0 ? col(gih, gjh, c.wxy[RY]) : -1;
-    if (c.hrow < 0)
-        c.wxy[RY] = 0;
+    c.colb[RY] = c.hrow >= 0 ? col(gih, gjh) : -1;
+    c.hwxy = 0;
+    if (c.colb[RY] >= 0)
+        c.hwxy = wall_count(g, 0, wrap_or_ghost(g, 0, gih)) +
+                 wall_count(g, 1, wrap_or_ghost(g, 1, gjh));
     return c;
 }
 
-// Linear index of slot s at global plane gk, or -1 for a zero ghost.
-__device__ __forceinline__ int64_t tile3_at(const Geo& g, const Tile3Ctx& c,
-                                            int s, int gk)
+// Plane-uniform data of global plane gk: offset of the plane (-1 for a zero
+// ghost plane), its z wall count, and its z couplings.
+struct Plane3Meta
 {
+    int64_t off;
+    int wz;
+    bool zwall_m, zwall_p;
+};
+
+__device__ __forceinline__ Plane3Meta plane3_meta(const Geo& g, int gk)
+{
+    Plane3Meta m;
     const int zc = wrap_or_ghost(g, 2, gk);
-    if (c.colb[s] < 0 || zc < 0)
-        return -1;
-    return c.colb[s] + g.px * g.py * (int64_t)(g.h + zc);
+    m.off = zc < 0 ? -1 : g.px * g.py * (int64_t)(g.h + zc);
+    m.wz = zc < 0 ? 0 : wall_count(g, 2, zc);
+    m.zwall_m = wall_lo(g, 2, gk);
+    m.zwall_p = wall_hi(g, 2, gk);
+    return m;
 }
 
 // Raw operands of one plane: a = r (direction) or p (update), b = old
@@ -160,96 +199,85 @@ struct Plane3
 };
 
 template <class T, bool DIR>
-__device__ __forceinline__ void tile3_load(const Geo& g, const Tile3Ctx& c,
-                                           int gk, const T* __restrict__ va,
+__device__ __forceinline__ void tile3_load(const Tile3Ctx& c, const Plane3Meta& m,
+                                           const T* __restrict__ va,
                                            const T* __restrict__ vb, bool need_b,
                                            Plane3<T, DIR>& P)
 {
 #pragma unroll
     for (int s = 0; s < tile3::NS; ++s) {
-        const int64_t at = tile3_at(g, c, s, gk);
-        P.a[s] = at >= 0 ? va[at] : (T)0;
+        const bool ok = c.colb[s] >= 0 && m.off >= 0;
+        const int64_t at = c.colb[s] + m.off;
+        P.a[s] = ok ? va[at] : (T)0;
         if constexpr (DIR)
-            P.b[s] = (at >= 0 && need_b) ? vb[at] : (T)0;
+            P.b[s] = (ok && need_b) ? vb[at] : (T)0;
     }
 }
 
-// Value of slot s at plane gk: the new direction fma(beta, p_old, D^-1 r)
-// (D^-1 r alone on the first iteration) for DIR, else the loaded p.
+// Value of slot s: the new direction fma(beta, p_old, D^-1 r) (D^-1 r
+// alone on the first iteration) for DIR, else the loaded p.
 template <class T, bool DIR>
 __device__ __forceinline__ T tile3_value(const Geo& g, const Tile3Ctx& c,
-                                         const Plane3<T, DIR>& P, int s, int gk,
+                                         const Plane3Meta& m,
+                                         const Plane3<T, DIR>& P, int s,
                                          bool first, T beta)
 {
     if constexpr (!DIR) {
         return P.a[s];
     } else {
-        const int zc = wrap_or_ghost(g, 2, gk);
-        const int wz = zc < 0 ? 0 : wall_count(g, 2, zc);
         T dinv;
-        if (s < tile3::RY) {
-            // own cell: the y and z parts are wave-uniform
-            const int gj = c.j0 + c.w * tile3::RY + s;
-            const int b = wrap_or_ghost(g, 1, gj);
-            const int wy = b < 0 ? 0 : wall_count(g, 1, b);
-            dinv = dinv_pick<T>(g, wy + wz, c.wx);
-        } else {
-            dinv = dinv_sel<T>(g, c.wxy[s] + wz);
-        }
+        if (s < tile3::RY)
+            dinv = dinv_pick<T>(g, c.wy[s] + m.wz, c.wx); // uniform y, z part
+        else
+            dinv = dinv_sel<T>(g, c.hwxy + m.wz);
         const T z = dinv * P.a[s];
         return first ? z : fma_t<T>(beta, P.b[s], z);
     }
 }
 
-// Writes plane gk's values into LDS plane pl and own[]; the halo slot fills
-// its ring cell.
+// Writes the plane's values into LDS plane pl and own[]; the halo slot
+// fills its ring cell.
 template <class T, bool DIR>
 __device__ __forceinline__ void tile3_finish(const Geo& g, const Tile3Ctx& c,
-                                             const Plane3<T, DIR>& P, int gk,
-                                             bool first, T beta,
-                                             T (*pl)[tile3::TX + 2],
+                                             const Plane3Meta& m,
+                                             const Plane3<T, DIR>& P, bool first,
+                                             T beta, T (*pl)[tile3::TX + 2],
                                              T (&own)[tile3::RY])
 {
     using namespace tile3;
 #pragma unroll
     for (int q = 0; q < RY; ++q) {
-        own[q] = tile3_value<T, DIR>(g, c, P, q, gk, first, beta);
+        own[q] = tile3_value<T, DIR>(g, c, m, P, q, first, beta);
         pl[c.w * RY + q + 1][c.lane + 1] = own[q];
     }
     if (c.hrow >= 0)
-        pl[c.hrow][c.hcol] = tile3_value<T, DIR>(g, c, P, RY, gk, first, beta);
+        pl[c.hrow][c.hcol] = tile3_value<T, DIR>(g, c, m, P, RY, first, beta);
 }
 
-// y = A p at own cell (row jj) of plane k: centre pc, z neighbours pm / pp,
+// y = A p at own cell (row q) of a plane: centre pc, z neighbours pm / pp,
 // x and y neighbours from the LDS plane; reference stencil order
 // {0}, -x, +x, -y, +y, -z, +z with zero couplings across walls.
 template <class T>
-__device__ __forceinline__ T tile3_apply(const Geo& g, const T (*pl)[tile3::TX + 2],
-                                         int lane, int jj, int gi, int gj, int gk,
+__device__ __forceinline__ T tile3_apply(const Geo& g, const Tile3Ctx& c,
+                                         const Plane3Meta& m,
+                                         const T (*pl)[tile3::TX + 2], int q,
                                          T pc, T pm, T pp)
 {
     const T s = offdiag<T>(g);
-    auto cf = [&](int d, int side, int gidx) -> T {
-        if (g.periodic[d])
-            return s;
-        const bool wall = side == 0 ? gidx <= 0 : gidx > g.n[d] - 2;
-        return wall ? (T)0 : s;
-    };
-    T y = diag_pick<T>(g, wall_count(g, 1, gj) + wall_count(g, 2, gk),
-                       wall_count(g, 0, gi)) *
-          pc;
-    y = fma_t<T>(cf(0, 0, gi), pl[jj + 1][lane], y);
-    y = fma_t<T>(cf(0, 1, gi), pl[jj + 1][lane + 2], y);
-    y = fma_t<T>(cf(1, 0, gj), pl[jj][lane + 1], y);
-    y = fma_t<T>(cf(1, 1, gj), pl[jj + 2][lane + 1], y);
-    y = fma_t<T>(cf(2, 0, gk), pm, y);
-    y = fma_t<T>(cf(2, 1, gk), pp, y);
+    const int jj = c.w * tile3::RY + q;
+    const int ln = c.lane;
+    T y = diag_pick<T>(g, c.wy[q] + m.wz, c.wx) * pc;
+    y = fma_t<T>(c.xwall_m ? (T)0 : s, pl[jj + 1][ln], y);
+    y = fma_t<T>(c.xwall_p ? (T)0 : s, pl[jj + 1][ln + 2], y);
+    y = fma_t<T>(c.ywall_m[q] ? (T)0 : s, pl[jj][ln + 1], y);
+    y = fma_t<T>(c.ywall_p[q] ? (T)0 : s, pl[jj + 2][ln + 1], y);
+    y = fma_t<T>(m.zwall_m ? (T)0 : s, pm, y);
+    y = fma_t<T>(m.zwall_p ? (T)0 : s, pp, y);
     return y;
 }
 
-// Direction + SpMV: p_new = fma(beta, p_old, D^-1 r); q = A p_new;
-// partial p_new.q.  UPDATE (DIR = false): q = A p; x += alpha p;
-// r -= alpha q; partials r.r, r.(D^-1 r).
+// The z march shared by the direction (DIR) and update kernels; epi.cell()
+// receives every owned cell's centre value and A p.
 template <class T, bool DIR, class Epi>
 __device__ __forceinline__ void tile3_march(const Geo& g, const Tile3Grid& tg,
                                             const T* __restrict__ va,
@@ -265,32 +293,32 @@ __device__ __forceinline__ void tile3_march(const Geo& g, const Tile3Grid& tg,
     Plane3<T, DIR> P0, P1;
     T pm[RY], pc[RY], pp[RY];
     // prologue: plane k0-1 (own column only), plane k0 into LDS, k0+1 in flight
-    tile3_load<T, DIR>(g, c, c.k0 - 1, va, vb, need_b, P0);
+    Plane3Meta mm = plane3_meta(g, c.k0 - 1);
+    tile3_load<T, DIR>(c, mm, va, vb, need_b, P0);
 #pragma unroll
     for (int q = 0; q < RY; ++q)
-        pm[q] = tile3_value<T, DIR>(g, c, P0, q, c.k0 - 1, first, beta);
-    tile3_load<T, DIR>(g, c, c.k0, va, vb, need_b, P1);
-    tile3_finish<T, DIR>(g, c, P1, c.k0, first, beta, pl[0], pc);
-    tile3_load<T, DIR>(g, c, c.k0 + 1, va, vb, need_b, P0);
-    epi.prefetch(c, c.k0);
+        pm[q] = tile3_value<T, DIR>(g, c, mm, P0, q, first, beta);
+    Plane3Meta mc = plane3_meta(g, c.k0);
+    tile3_load<T, DIR>(c, mc, va, vb, need_b, P1);
+    tile3_finish<T, DIR>(g, c, mc, P1, first, beta, pl[0], pc);
+    Plane3Meta mn = plane3_meta(g, c.k0 + 1);
+    tile3_load<T, DIR>(c, mn, va, vb, need_b, P0);
+    epi.prefetch(c, mc, c.k0);
     epi.rotate();
     __syncthreads();
     int cur = 0;
     for (int k = c.k0; k < c.k1; ++k) {
         // P0 holds plane k+1 (loaded last step); start plane k+2
+        const Plane3Meta mnn = plane3_meta(g, k + 2);
         if (k + 2 <= c.k1)
-            tile3_load<T, DIR>(g, c, k + 2, va, vb, need_b, P1);
-        tile3_finish<T, DIR>(g, c, P0, k + 1, first, beta, pl[cur ^ 1], pp);
-        epi.prefetch(c, k + 1);
+            tile3_load<T, DIR>(c, mnn, va, vb, need_b, P1);
+        tile3_finish<T, DIR>(g, c, mn, P0, first, beta, pl[cur ^ 1], pp);
+        epi.prefetch(c, mn, k + 1);
 #pragma unroll
         for (int q = 0; q < RY; ++q) {
-            const int jj = c.w * RY + q;
-            const int gi = c.i0 + c.lane;
-            const int gj = c.j0 + jj;
-            if (gi < g.n[0] && gj < g.n[1]) {
-                const T y = tile3_apply<T>(g, pl[cur], c.lane, jj, gi, gj, k,
-                                           pc[q], pm[q], pp[q]);
-                epi.cell(c, q, gi, gj, k, pc[q], y);
+            if (c.valid[q]) {
+                const T y = tile3_apply<T>(g, c, mc, pl[cur], q, pc[q], pm[q], pp[q]);
+                epi.cell(c, mc, q, pc[q], y);
             }
             pm[q] = pc[q];
             pc[q] = pp[q];
@@ -298,6 +326,8 @@ __device__ __forceinline__ void tile3_march(const Geo& g, const Tile3Grid& tg,
         __syncthreads();
         cur ^= 1;
         P0 = P1;
+        mc = mn;
+        mn = mnn;
         epi.rotate();
     }
 }

@@ -219,7 +219,9 @@ class Solver:
 
     # ---- measurement -----------------------------------------------------------
     def set_kernel_timing(self, on=True):
-        self._call(self.L.cf_set_kernel_timing, 1 if on else 0, what="set_kernel_timing")
+        """on: False/0 off; True/1 every PCG iteration; N > 1 one in N."""
+        n = int(on) if not isinstance(on, bool) else (1 if on else 0)
+        self._call(self.L.cf_set_kernel_timing, n, what="set_kernel_timing")
 
     def kernel_time(self, kernel):
         ms = C.c_double()

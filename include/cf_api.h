@@ -179,8 +179,9 @@ cf_status cf_init_blobs(cf_solver* h, uint64_t seed, int nblobs);
 cf_status cf_init_random_faces(cf_solver* h, uint64_t seed);
 
 /* ---- measurement -------------------------------------------------------- */
-/* When enabled, the PCG brackets each launch of its kernels with HIP events
- * on the handle's stream and accumulates per-kernel device time. */
+/* enable > 0: bracket kernel launches with HIP events on the handle's stream
+ * and accumulate per-kernel device time; the PCG kernels are timed in one
+ * iteration out of every `enable` (1: every iteration).  0: off. */
 cf_status cf_set_kernel_timing(cf_solver* h, int enable);
 /* kernel: 0 FP64 direction+SpMV, 1 FP64 update, 2 FP64 finalize kernels,
  * 3-5 the same for the FP32 inner solve, 6 advection, 7 FP64
