@@ -4,6 +4,7 @@
 
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "cf_solver.hpp"
 
@@ -403,10 +404,41 @@ cf_status cf_adjoint_run(cf_solver* h, int nsteps, int ckpt,
                          double* dv0_host, double* dw0_host, double* dq0_host,
                          double dvel[3])
 {
-    return guarded(h, [&](cfmi::Solver&) -> cf_status {
-        (void)nsteps; (void)ckpt; (void)dq_T_host; (void)du0_host;
-        (void)dv0_host; (void)dw0_host; (void)dq0_host; (void)dvel;
-        throw CfError(CF_ENOTIMPL, "adjoint not built yet");
+    return guarded(h, [&](cfmi::Solver& s) -> cf_status {
+        if (!dq_T_host || !dvel)
+            throw CfError(CF_EINVAL, "dq_T and dvel are required");
+        const cfmi::Geo& g = s.geo();
+        // dq_T with every non-owned entry cleared: the loss is defined on
+        // the owned cells
+        const int* E = g.ext[CF_CELL];
+        const size_t ncell = (size_t)E[0] * E[1] * E[2];
+        std::vector<double> dq(dq_T_host, dq_T_host + ncell);
+        for (int k = 0; k < E[2]; ++k)
+            for (int j = 0; j < E[1]; ++j)
+                for (int i = 0; i < E[0]; ++i) {
+                    const bool own = i >= g.h && i < g.own_hi[0][0] && j >= g.h &&
+                                     j < g.own_hi[0][1] && k >= g.own_lo(2) &&
+                                     k < g.own_hi[0][2];
+                    if (!own)
+                        dq[((size_t)k * E[1] + j) * E[0] + i] = 0.0;
+                }
+        cfmi::DevArray<double> dq_dev, gr[4];
+        dq_dev.alloc(g.total, s.stream());
+        copy_dense(s, dq_dev.ptr, CF_CELL, dq.data(), true);
+        double* host_out[4] = {dq0_host, du0_host, dv0_host, dw0_host};
+        double* dev_out[4] = {nullptr, nullptr, nullptr, nullptr};
+        for (int e = 0; e <= g.dim; ++e)
+            if (host_out[e]) {
+                gr[e].alloc(g.total, s.stream());
+                dev_out[e] = gr[e].ptr;
+            }
+        cf_status rc = s.adjointRun(nsteps, ckpt, dq_dev.ptr, dev_out, dvel);
+        if (rc != CF_OK)
+            return rc;
+        for (int e = 0; e <= g.dim; ++e)
+            if (host_out[e])
+                copy_dense(s, gr[e].ptr, e, host_out[e], false);
+        return CF_OK;
     });
 }
 

@@ -152,6 +152,8 @@ class VelocityCorrector
     void applyPressure();
     double* lhs() const { return _lhs.ptr; }
     double* rhs() const { return _rhs.ptr; }
+    RedBuf redBuf() const { return RedBuf{_red_partials.ptr, _red_counter.ptr}; }
+    double* scratch() const { return _scratch.ptr; }
 
   private:
     cf_status solve64(const double* b, double* x, cf_solve_stats* st);
@@ -205,6 +207,13 @@ class Solver
     KernelTimer& timer() { return _timer; }
     double time() const { return _time; }
     std::string last_error;
+
+    // Reverse mode of setup() + nsteps step() from the state in Current
+    // (cf_adjoint_run.cpp).  dq_T: dL/dq of the final state (device, padded
+    // layout, ghost entries ignored); grad[e]: device arrays that receive
+    // dL/d(initial entity e) (owned entries; ghosts zero); dvel: host.
+    cf_status adjointRun(int nsteps, int ckpt, const double* dq_T,
+                         double* const* grad, double dvel[3]);
 
     void initVortexTubes(uint64_t seed, double peak);
     void initBlobs(uint64_t seed, int nblobs);

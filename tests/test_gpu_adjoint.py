@@ -1,0 +1,299 @@
+"""The HIP adjoint (cf_adjoint_run) against the oracle's FP64 reverse mode
+and against central finite differences of the HIP forward.
+
+The oracle's reverse mode is itself pinned by finite differences of the
+oracle's forward (test_oracle_adjoint.py).  Scatters in the HIP adjoint are
+FP64 atomics, so the bar is the 1e-6 relative gradient tolerance, not bit
+equality.
+"""
+import numpy as np
+import pytest
+
+from cajitafluids_amd import configs as K
+from helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+GRAD_TOL = 1e-6
+
+
+@pytest.fixture(scope="module")
+def cf():
+    from cajitafluids_amd import solver
+    return solver
+
+
+def _random_state(cf, cfg, seed, amp=0.3):
+    rng = np.random.default_rng(seed)
+    out = {}
+    for e in range(cfg["dim"] + 1):
+        ext, lo, hi = cf.layout(cfg, e)
+        a = np.zeros((ext[2], ext[1], ext[0]))
+        sl = (slice(lo[2], hi[2]), slice(lo[1], hi[1]), slice(lo[0], hi[0]))
+        a[sl] = rng.normal(size=a[sl].shape) * (amp if e else 1.0)
+        out[e] = a
+    return out
+
+
+def _owned(cf, cfg, e, a):
+    _, lo, hi = cf.layout(cfg, e)
+    return a[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]]
+
+
+def _gpu_loss(cf, cfg, init, nsteps, w):
+    with cf.Solver(cfg) as sol:
+        for e, a in init.items():
+            sol.set(e, a)
+        sol.setup()
+        for _ in range(nsteps):
+            sol.step()
+        return float(np.sum(w * sol.get(0)))
+
+
+def _gpu_adjoint(cf, cfg, init, nsteps, ckpt, dqT):
+    with cf.Solver(cfg) as sol:
+        for e, a in init.items():
+            sol.set(e, a)
+        g = sol.adjoint_run(nsteps, ckpt, dqT)
+        final = {e: sol.get(e) for e in init}
+    return g, final
+
+
+def _case(name):
+    tank = K.adjoint_tank_2d(24)
+    tank.update(cg_tol=1e-12, inflow_velocity=[0.5, 3.0, 0.0])
+    if name == "2d_linear_quirks":
+        return tank, 5, 2
+    if name == "2d_cubic_cubic":
+        c = K.copy_config(tank)
+        c.update(field_order=3, velocity_order=3)
+        return c, 5, 3
+    if name == "2d_no_quirks":
+        c = K.copy_config(tank)
+        c["quirks"] = 0
+        return c, 4, 1
+    if name == "2d_periodic_x":
+        c = K.copy_config(tank)
+        c["periodic"] = [1, 0, 0]
+        return c, 4, 2
+    if name == "2d_periodic_xy_cubic":
+        c = K.copy_config(tank)
+        c.update(periodic=[1, 1, 0], field_order=3)
+        return c, 4, 4
+    if name == "2d_free_wall":
+        c = K.copy_config(tank)
+        c["wall_type"] = [[K.SOLID, K.FREE], [K.SOLID, K.SOLID], [K.SOLID, K.SOLID]]
+        return c, 4, 2
+    if name == "3d_periodic_xy_cubic":
+        c = K.periodic_box_3d(10)
+        c.update(cg_tol=1e-12, precision=K.FP64, inflow_shape=K.INFLOW_SPHERE,
+                 inflow_center=[0.5, 0.5, 0.3], inflow_radius=0.25,
+                 inflow_velocity=[0.3, -0.2, 2.0], inflow_quantity=1.0,
+                 clamp_dt=0, dt=0.01)
+        return c, 3, 2
+    if name == "3d_tank_quirks":
+        c = K.vortex_tank_3d(10)
+        c.update(cg_tol=1e-12, clamp_dt=0, dt=0.01, quirks=3, inflow_radius=0.3,
+                 inflow_velocity=[0.3, 0.2, 2.0])
+        return c, 3, 2
+    if name == "3d_tank_fp32_refined":
+        c = K.vortex_tank_3d(12, precision=K.FP32_REFINED)
+        c.update(cg_tol=1e-10, clamp_dt=0, dt=0.01, inflow_radius=0.3,
+                 inflow_velocity=[0.3, 0.2, 2.0])
+        return c, 3, 1
+    raise KeyError(name)
+
+
+CASES = ["2d_linear_quirks", "2d_cubic_cubic", "2d_no_quirks", "2d_periodic_x",
+         "2d_periodic_xy_cubic", "2d_free_wall", "3d_periodic_xy_cubic",
+         "3d_tank_quirks", "3d_tank_fp32_refined"]
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_adjoint_matches_oracle(cf, oracle_mod, name):
+    cfg, nsteps, ckpt = _case(name)
+    print(f"adjoint vs oracle: {name} steps={nsteps} ckpt={ckpt}", flush=True)
+    init = _random_state(cf, cfg, seed=7)
+    w = _random_state(cf, cfg, seed=8)[0]
+    g, final = _gpu_adjoint(cf, cfg, init, nsteps, ckpt, w)
+
+    ocfg = K.copy_config(cfg)
+    ocfg["precision"] = K.FP64
+    orc = oracle_mod.Oracle(ocfg)
+    for e, a in init.items():
+        orc.field(e)[...] = a
+    rc, go = orc.adjoint_run(nsteps, ckpt, w)
+    assert rc == 0
+    keys = {0: "dq0", 1: "du", 2: "dv", 3: "dw"}
+    for e in range(cfg["dim"] + 1):
+        a = _owned(cf, cfg, e, g[keys[e]])
+        b = _owned(cf, cfg, e, go[keys[e]])
+        err = rel_l2(a, b)
+        assert err <= GRAD_TOL, f"{name} entity {e}: rel L2 {err:.3e}"
+        # ghost entries of a gradient are zero
+        assert np.count_nonzero(g[keys[e]]) == np.count_nonzero(a)
+    dv_ref = go["dvel"]
+    assert np.allclose(g["dvel"], dv_ref, rtol=GRAD_TOL,
+                       atol=GRAD_TOL * max(1.0, float(np.abs(dv_ref).max())))
+    # the handle is left holding the final forward state
+    for e in range(cfg["dim"] + 1):
+        ref = orc.owned_view(orc.field(e), e)
+        assert rel_l2(_owned(cf, cfg, e, final[e]), ref) <= 1e-9
+    orc.close()
+
+
+def test_adjoint_setup_only(cf, oracle_mod):
+    cfg, _, _ = _case("2d_linear_quirks")
+    init = _random_state(cf, cfg, seed=3)
+    w = _random_state(cf, cfg, seed=4)[0]
+    g, _ = _gpu_adjoint(cf, cfg, init, 0, 1, w)
+    orc = oracle_mod.Oracle(cfg)
+    for e, a in init.items():
+        orc.field(e)[...] = a
+    rc, go = orc.adjoint_run(0, 1, w)
+    assert rc == 0
+    for e, k in ((0, "dq0"), (1, "du"), (2, "dv")):
+        assert rel_l2(_owned(cf, cfg, e, g[k]), _owned(cf, cfg, e, go[k])) <= GRAD_TOL
+    orc.close()
+
+
+def test_adjoint_checkpoint_invariance(cf):
+    """The gradient does not depend on the checkpoint interval."""
+    cfg, _, _ = _case("2d_linear_quirks")
+    init = _random_state(cf, cfg, seed=11)
+    w = _random_state(cf, cfg, seed=12)[0]
+    ref, _ = _gpu_adjoint(cf, cfg, init, 7, 7, w)
+    for ck in (1, 3):
+        g, _ = _gpu_adjoint(cf, cfg, init, 7, ck, w)
+        for k in ("dq0", "du", "dv"):
+            assert rel_l2(g[k], ref[k]) <= 1e-10, (ck, k)
+
+
+@pytest.mark.parametrize("name", ["2d_linear_quirks", "3d_periodic_xy_cubic"])
+def test_adjoint_vs_finite_differences(cf, name):
+    cfg, nsteps, ckpt = _case(name)
+    init = _random_state(cf, cfg, seed=21)
+    w = _random_state(cf, cfg, seed=22)[0]
+    g, _ = _gpu_adjoint(cf, cfg, init, nsteps, ckpt, w)
+    direction = _random_state(cf, cfg, seed=23, amp=1.0)
+    keys = {0: "dq0", 1: "du", 2: "dv", 3: "dw"}
+    pred = sum(float(np.sum(g[keys[e]] * direction[e])) for e in init)
+    eps = 1e-5
+
+    def floor(lp, lm):
+        # rounding noise of a central difference of two O(|L|) numbers
+        return 1e-14 * max(abs(lp), abs(lm)) / eps
+    lp = _gpu_loss(cf, cfg, {e: init[e] + eps * direction[e] for e in init}, nsteps, w)
+    lm = _gpu_loss(cf, cfg, {e: init[e] - eps * direction[e] for e in init}, nsteps, w)
+    fd = (lp - lm) / (2 * eps)
+    assert abs(pred - fd) <= GRAD_TOL * abs(fd) + floor(lp, lm), (pred, fd)
+    for d in range(cfg["dim"]):
+        cp = K.copy_config(cfg)
+        cp["inflow_velocity"][d] += eps
+        cm = K.copy_config(cfg)
+        cm["inflow_velocity"][d] -= eps
+        lp, lm = _gpu_loss(cf, cp, init, nsteps, w), _gpu_loss(cf, cm, init, nsteps, w)
+        fdu = (lp - lm) / (2 * eps)
+        assert abs(g["dvel"][d] - fdu) <= GRAD_TOL * abs(fdu) + floor(lp, lm), \
+            (d, g["dvel"][d], fdu)
+
+
+def test_dye_tank_adjoint_512(cf):
+    """The 2-D 512^2 dye-tank adjoint: 100 steps, checkpoints every 10, loss
+    0.5 ||q_T - target||^2 against 8 seeded Gaussian blobs; gradient in a
+    random direction of the initial velocity against central differences."""
+    cfg = K.adjoint_tank_2d(512)
+    cfg["cg_tol"] = 1e-11
+    nsteps, ckpt = K.CONFIG4_STEPS, K.CONFIG4_CKPT
+    with cf.Solver(cfg) as s:
+        s.init_blobs(K.CONFIG4_TARGET_SEED, 8)
+        target = s.get(0)
+    init = {e: np.zeros(tuple(reversed(cf.layout(cfg, e)[0]))) for e in range(3)}
+    rng = np.random.default_rng(K.CONFIG4_FD_SEED)
+    for e in (1, 2):
+        _, lo, hi = cf.layout(cfg, e)
+        sl = (slice(lo[2], hi[2]), slice(lo[1], hi[1]), slice(lo[0], hi[0]))
+        init[e][sl] = 0.2 * rng.normal(size=init[e][sl].shape)
+
+    def loss(c, st):
+        with cf.Solver(c) as s:
+            for e, a in st.items():
+                s.set(e, a)
+            s.setup()
+            for _ in range(nsteps):
+                s.step()
+            q = _owned(cf, c, 0, s.get(0))
+        return 0.5 * float(np.sum((q - _owned(cf, c, 0, target)) ** 2)), q
+
+    l0, qT = loss(cfg, init)
+    dqT = np.zeros_like(target)
+    _, lo, hi = cf.layout(cfg, 0)
+    dqT[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]] = qT - _owned(cf, cfg, 0, target)
+    g, _ = _gpu_adjoint(cf, cfg, init, nsteps, ckpt, dqT)
+    direction = {e: np.zeros_like(init[e]) for e in init}
+    for e in (1, 2):
+        _, lo, hi = cf.layout(cfg, e)
+        sl = (slice(lo[2], hi[2]), slice(lo[1], hi[1]), slice(lo[0], hi[0]))
+        direction[e][sl] = rng.normal(size=direction[e][sl].shape)
+    pred = float(np.sum(g["du"] * direction[1]) + np.sum(g["dv"] * direction[2]))
+    eps = 1e-5
+    lp, _ = loss(cfg, {e: init[e] + eps * direction[e] for e in init})
+    lm, _ = loss(cfg, {e: init[e] - eps * direction[e] for e in init})
+    fd = (lp - lm) / (2 * eps)
+    floor = 1e-14 * max(abs(lp), abs(lm)) / eps
+    print(f"dye tank 512: L={l0:.6e} adjoint={pred:.10e} fd={fd:.10e}", flush=True)
+    assert abs(pred - fd) <= GRAD_TOL * abs(fd) + floor, (pred, fd)
+    # inflow speed (the dt clamp is off in this workload: dt is fixed)
+    cp = K.copy_config(cfg)
+    cp["inflow_velocity"][1] += eps
+    cm = K.copy_config(cfg)
+    cm["inflow_velocity"][1] -= eps
+    lp, lm = loss(cp, init)[0], loss(cm, init)[0]
+    fdu = (lp - lm) / (2 * eps)
+    floor = 1e-14 * max(abs(lp), abs(lm)) / eps
+    print(f"dye tank 512: dL/dU_y adjoint={g['dvel'][1]:.10e} fd={fdu:.10e}", flush=True)
+    assert abs(g["dvel"][1] - fdu) <= GRAD_TOL * abs(fdu) + floor, (g["dvel"][1], fdu)
+
+
+def test_autograd_rollout(cf):
+    """torch.autograd through the rollout: backward equals the C-ABI adjoint
+    and a finite difference of the forward."""
+    import torch
+    from cajitafluids_amd.autograd import owned_shape, rollout
+
+    cfg, nsteps, ckpt = _case("2d_linear_quirks")
+    dev = "cuda"
+    rng = np.random.default_rng(31)
+    t = {e: torch.tensor(rng.normal(size=owned_shape(cfg, e)) * (0.3 if e else 1.0),
+                         dtype=torch.float64, device=dev, requires_grad=True)
+         for e in range(3)}
+    U = torch.tensor(cfg["inflow_velocity"], dtype=torch.float64, device=dev,
+                     requires_grad=True)
+    wt = torch.tensor(rng.normal(size=owned_shape(cfg, 0)), dtype=torch.float64, device=dev)
+    qT = rollout(cfg, nsteps, t[0], t[1], t[2], None, U, ckpt=ckpt)
+    loss = (wt * qT).sum()
+    loss.backward()
+    # C-ABI adjoint on the same inputs
+    init = {}
+    for e in range(3):
+        ext, lo, hi = cf.layout(cfg, e)
+        a = np.zeros((ext[2], ext[1], ext[0]))
+        a[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]] = t[e].detach().cpu().numpy()
+        init[e] = a
+    w = np.zeros_like(init[0])
+    _, lo, hi = cf.layout(cfg, 0)
+    w[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]] = wt.cpu().numpy()
+    g, _ = _gpu_adjoint(cf, cfg, init, nsteps, ckpt, w)
+    for e, k in ((0, "dq0"), (1, "du"), (2, "dv")):
+        assert rel_l2(t[e].grad.cpu().numpy(), _owned(cf, cfg, e, g[k])) <= 1e-12
+    assert np.allclose(U.grad.cpu().numpy(), g["dvel"], rtol=1e-12, atol=1e-14)
+    # finite difference along u0
+    d = torch.tensor(rng.normal(size=owned_shape(cfg, 1)), dtype=torch.float64, device=dev)
+    eps = 1e-5
+    with torch.no_grad():
+        lp = float((wt * rollout(cfg, nsteps, t[0], t[1] + eps * d, t[2], None, U)).sum())
+        lm = float((wt * rollout(cfg, nsteps, t[0], t[1] - eps * d, t[2], None, U)).sum())
+    fd = (lp - lm) / (2 * eps)
+    pred = float((t[1].grad * d).sum())
+    floor = 1e-14 * max(abs(lp), abs(lm)) / eps
+    assert abs(pred - fd) <= GRAD_TOL * abs(fd) + floor, (pred, fd)
