@@ -43,6 +43,12 @@ def parse():
     ap.add_argument("--timing-sample", type=int, default=8,
                     help="time the PCG kernels with HIP events in one "
                          "iteration out of N inside the timed region (0: off)")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="also time the reverse mode: the 2-D 512^2 dye tank (100 "
+                         "steps, checkpoint every 10) and the 3-D 384^3 periodic "
+                         "box (10 steps, checkpoint every 2) against their forwards")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the reverse-mode leg (profiling runs)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a plain 1 GiB device copy, the reference "
                          "kernel for scaling rocprof FETCH_SIZE / WRITE_SIZE")
@@ -70,7 +76,6 @@ def make_solver(cfg):
 
 
 def timed_run(sol, steps, warmup, timing=True):
-    from cajitafluids_amd import _lib
     sol.setup()
     for _ in range(warmup):
         sol.step()
@@ -100,6 +105,9 @@ def main():
         sys.exit(2)
     from cajitafluids_amd import _lib, configs as K
 
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_legs()}))
+        return
     if args.copy_probe:
         copy_probe()
     n = args.n
@@ -115,6 +123,7 @@ def main():
     t64, it64, _, res64, kt = timed_run(sol, args.steps, args.warmup,
                                         timing=args.timing_sample)
     dt_eff = sol.dt
+    q64, p64 = owned_q_and_p(sol) if not args.no_fp32 else (None, None)
     sol.close()
     del sol
 
@@ -126,7 +135,6 @@ def main():
         if cnt:
             per[nm] = dict(avg_us=1e3 * ms / cnt, launches=cnt, total_ms=ms,
                            bytes=WORDS[nm] * 8 * ncell)
-    fin_ms, fin_n = kt[_lib.TK_FINALIZE]
     # Kernel time per iteration from the sampled launches, times the
     # iterations of the timed region.
     t_pcg = sum(p["avg_us"] for p in per.values()) * 1e-6 * sum(it64)
@@ -138,9 +146,12 @@ def main():
         dom = max(per, key=lambda k: per[k]["avg_us"])
         d = per[dom]
         achieved = d["bytes"] / (d["avg_us"] * 1e-6) / 1e9
-        roofline = {"bound": "hbm", "kernel": f"k_pcg_{dom}", "achieved": round(achieved, 1),
+        kname = f"k_pcg_{dom}3<double>" if n == 256 else None
+        traffic = measured_traffic(kname) if kname else None
+        roofline = {"bound": "hbm", "kernel": f"k_pcg_{dom}3<double>", "achieved": round(achieved, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "traffic_note": TRAFFIC_NOTE if traffic else None,
                     "bytes_per_launch": d["bytes"], "avg_us": round(d["avg_us"], 2)}
     adv_ms, adv_n = kt[_lib.TK_ADVECT]
 
@@ -164,7 +175,6 @@ def main():
                 "iterations": it64, "gbs_11_words": round(pcg_gbs, 1) if pcg_gbs else None,
                 "frac_11_words": round(pcg_gbs / HBM_PEAK_GBS, 4) if pcg_gbs else None,
                 "time_share": round(t_pcg / t64, 4),
-                "finalize_avg_us": round(1e3 * fin_ms / fin_n, 2) if fin_n else None,
                 "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv)
                                 for kk, vv in v.items()} for k, v in per.items()},
                 "max_final_residual": max(res64)},
@@ -177,6 +187,7 @@ def main():
         s32 = make_solver(cfg32)
         t32, it32, outer32, res32, kt32 = timed_run(s32, args.steps, args.warmup,
                                                     timing=args.timing_sample)
+        q32, p32 = owned_q_and_p(s32)
         s32.close()
         del s32
         v32 = ncell * args.steps / t32
@@ -198,13 +209,120 @@ def main():
             "max_final_residual": max(res32),
             "refine_mode": "carry" if cfg32["refine_mode"] == K.REFINE_CARRY else "restart",
             "inner_rtol": cfg32["inner_tol"],
+            "dye_rel_l2_vs_fp64": rel(q32, q64),
+            "pressure_rel_l2_vs_fp64": rel(p32, p64),
             "kernels": k32,
         }
+
+    if args.adjoint:
+        out["adjoint"] = adjoint_legs()
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, n, total_iters / args.steps, args.cpu_iters)
 
     print(json.dumps(out))
+
+
+def adjoint_leg(cfg, nsteps, ckpt, init, loss_grad):
+    """Wall time of setup + nsteps (forward) and of cf_adjoint_run (which
+    runs the forward again with checkpoints, recomputes each segment and
+    sweeps back), from the same initial state."""
+    import numpy as np
+    from cajitafluids_amd.solver import Solver
+    with Solver(cfg) as s:
+        init(s)
+        s.sync()
+        t0 = time.perf_counter()
+        s.setup()
+        for _ in range(nsteps):
+            s.step()
+        s.sync()
+        t_fwd = time.perf_counter() - t0
+        dqT = loss_grad(s)
+    with Solver(cfg) as s:
+        init(s)
+        s.sync()
+        t0 = time.perf_counter()
+        g = s.adjoint_run(nsteps, ckpt, dqT)
+        t_adj = time.perf_counter() - t0
+    cells = int(np.prod([c for c in cfg["num_cell"][:cfg["dim"]]]))
+    return {"cells": cells, "steps": nsteps, "ckpt": ckpt,
+            "forward_s": round(t_fwd, 3), "adjoint_s": round(t_adj, 3),
+            "adjoint_over_forward": round(t_adj / t_fwd, 2),
+            "grad_u_norm": float(np.linalg.norm(g["du"])),
+            "note": "adjoint_s includes the host copies of dL/dq_T in and of the "
+                    "gradients out (cf_adjoint_run takes host arrays)"}
+
+
+def adjoint_legs():
+    import numpy as np
+    from cajitafluids_amd import configs as K
+    out = {}
+    # 2-D 512^2 dye tank: loss 0.5 ||q_T - blobs||^2, initial velocity 0.2 N(0,1)
+    c4 = K.adjoint_tank_2d(512)
+    from cajitafluids_amd.solver import Solver
+    with Solver(c4) as s:
+        s.init_blobs(K.CONFIG4_TARGET_SEED, 8)
+        target = s.get(0)
+    rng = np.random.default_rng(K.CONFIG4_FD_SEED)
+    faces = {}
+    for e in (1, 2):
+        from cajitafluids_amd.solver import layout
+        ext, lo, hi = layout(c4, e)
+        a = np.zeros((ext[2], ext[1], ext[0]))
+        sl = (slice(lo[2], hi[2]), slice(lo[1], hi[1]), slice(lo[0], hi[0]))
+        a[sl] = 0.2 * rng.normal(size=a[sl].shape)
+        faces[e] = a
+
+    def init4(s):
+        for e, a in faces.items():
+            s.set(e, a)
+    out["dye_tank_512"] = adjoint_leg(c4, K.CONFIG4_STEPS, K.CONFIG4_CKPT, init4,
+                                      lambda s: s.get(0) - target)
+    # 3-D 384^3 periodic box: vortex tubes (seed 5), dye blobs (seed 55),
+    # loss 0.5 ||q_T||^2
+    c5 = K.periodic_box_3d(384)
+
+    def init5(s):
+        s.init_vortex_tubes(K.CONFIG5_VEL_SEED, 1.0)
+        s.init_blobs(K.CONFIG5_DYE_SEED, 8)
+    out["periodic_box_384"] = adjoint_leg(c5, K.CONFIG5_STEPS, K.CONFIG5_CKPT, init5,
+                                          lambda s: s.get(0))
+    return out
+
+
+TRAFFIC_NOTE = ("bytes per launch = 2 x FETCH_SIZE + WRITE_SIZE from rocprofv3 --pmc "
+                "passes over this script at 256^3 (FETCH_SIZE reads half of a 1 GiB "
+                "copy on this GPU, WRITE_SIZE reads it whole); "
+                "profiles/pcg_tiles_vortex_tank_256.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel, from the counter summary committed
+    beside the kernel-trace summary (None when absent)."""
+    path = os.path.join(REPO, "profiles", "pcg_tiles_vortex_tank_256.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) == 4 and cells[0].startswith(kernel + " ") and cells[2].endswith("MB"):
+                    return round(float(cells[2].split()[0]) * 1e6)
+    except OSError:
+        pass
+    return None
+
+
+def owned_q_and_p(sol):
+    """Owned dye and last pressure (mean removed) of a solver, on the host."""
+    q = sol.owned_view(sol.get(0), 0).copy()
+    p = sol.owned_view(sol.get_pressure(0), 0).copy()
+    return q, p - p.mean()
+
+
+def rel(a, b):
+    import numpy as np
+    nb = float(np.linalg.norm(b))
+    return float(np.linalg.norm(a - b)) / nb if nb else float(np.linalg.norm(a))
 
 
 def cpu_baseline(cfg, n, iters_per_step, cpu_iters):

@@ -1,0 +1,120 @@
+"""Writes dye_tank_adjoint_512.json: the oracle's FP64 reverse mode of the
+2-D 512^2 dye-tank workload (setup + 100 steps, checkpoints every 10),
+loss 0.5 ||q_T - target||^2 against 8 seeded Gaussian blobs, initial
+velocity 0.2 N(0, 1) per owned face (numpy default_rng, seed 44).
+
+The fixture holds scalars only -- the loss, the inflow-velocity gradient,
+the L2 norms of the initial-state gradients, two directional derivatives
+and 64 sampled gradient entries -- so that the HIP adjoint can be checked
+at the full workload size without running the oracle (about 8 minutes on
+8 cores) in the test.
+
+    OMP_NUM_THREADS=8 python tests/golden/make_dye_tank_adjoint.py
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+import oracle_lib as O  # noqa: E402
+from cajitafluids_amd import configs as K  # noqa: E402
+
+OUT = os.path.join(HERE, "dye_tank_adjoint_512.json")
+N = 512
+INIT_SEED = 44
+SAMPLE_SEED = 45
+NSAMPLE = 64
+
+
+def workload():
+    return K.adjoint_tank_2d(N), K.CONFIG4_STEPS, K.CONFIG4_CKPT
+
+
+def initial_state(orc):
+    """Owned faces 0.2 N(0,1), faces drawn u then v; q = 0."""
+    rng = np.random.default_rng(INIT_SEED)
+    init = {e: np.zeros_like(orc.field(e)) for e in range(3)}
+    for e in (1, 2):
+        v = orc.owned_view(init[e], e)
+        v[...] = 0.2 * rng.normal(size=v.shape)
+    return init
+
+
+def directions(orc):
+    """Two unit-variance directions over the owned faces of u and v."""
+    rng = np.random.default_rng(INIT_SEED + 100)
+    out = []
+    for _ in range(2):
+        d = {e: np.zeros_like(orc.field(e)) for e in range(3)}
+        for e in (1, 2):
+            v = orc.owned_view(d[e], e)
+            v[...] = rng.normal(size=v.shape)
+        out.append(d)
+    return out
+
+
+def sample_indices(orc, entity):
+    """NSAMPLE flat indices into the owned region of an entity (z, y, x)."""
+    rng = np.random.default_rng(SAMPLE_SEED + entity)
+    shape = orc.owned_view(orc.field(entity), entity).shape
+    return rng.integers(0, int(np.prod(shape)), size=NSAMPLE).tolist()
+
+
+def main():
+    cfg, nsteps, ckpt = workload()
+    tgt = O.Oracle(cfg)
+    target = tgt.init_blobs(K.CONFIG4_TARGET_SEED, 8)
+    tv = tgt.owned_view(target, 0).copy()
+    tgt.close()
+
+    orc = O.Oracle(cfg)
+    init = initial_state(orc)
+    for e, a in init.items():
+        orc.field(e)[...] = a
+    rc, _ = orc.setup()
+    assert rc == 0
+    rc, _ = orc.run(nsteps)
+    assert rc == 0
+    qT = orc.owned_view(orc.field(0), 0).copy()
+    orc.close()
+    loss = 0.5 * float(np.sum((qT - tv) ** 2))
+
+    orc = O.Oracle(cfg)
+    for e, a in init.items():
+        orc.field(e)[...] = a
+    dqT = np.zeros_like(orc.field(0))
+    orc.owned_view(dqT, 0)[...] = qT - tv
+    rc, g = orc.adjoint_run(nsteps, ckpt, dqT)
+    assert rc == 0
+    keys = {0: "dq0", 1: "du", 2: "dv"}
+    own = {e: orc.owned_view(g[keys[e]], e) for e in range(3)}
+    dirs = directions(orc)
+    fixture = {
+        "workload": "2-D 512^2 dye tank, setup + 100 steps, checkpoint every 10, "
+                    "loss 0.5 ||q_T - blobs||^2",
+        "init_seed": INIT_SEED, "sample_seed": SAMPLE_SEED,
+        "target_seed": K.CONFIG4_TARGET_SEED, "nblobs": 8,
+        "nsteps": nsteps, "ckpt": ckpt, "cg_tol": cfg["cg_tol"],
+        "loss": loss,
+        "dvel": [float(x) for x in g["dvel"]],
+        "norm": {keys[e]: float(np.linalg.norm(own[e])) for e in range(3)},
+        "directional": [
+            float(sum(np.sum(g[keys[e]] * d[e]) for e in (1, 2))) for d in dirs],
+        "samples": {keys[e]: {"index": sample_indices(orc, e),
+                              "value": [float(own[e].reshape(-1)[i])
+                                        for i in sample_indices(orc, e)]}
+                    for e in range(3)},
+    }
+    orc.close()
+    with open(OUT, "w") as f:
+        json.dump(fixture, f, indent=1)
+    print("wrote", OUT)
+
+
+if __name__ == "__main__":
+    main()
