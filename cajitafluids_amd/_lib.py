@@ -151,6 +151,22 @@ class CfError(RuntimeError):
         self.code = code
 
 
+def _torch_runtime_first():
+    """The torch wheel carries its own HIP/HSA runtime next to the system one
+    that libcfmi.so links.  Both can serve one process only if torch's
+    initialises first: in the other order torch finds no device.  So when
+    torch is importable, its runtime is brought up before libcfmi.so loads."""
+    try:
+        import torch
+    except Exception:
+        return
+    try:
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except Exception:
+        pass
+
+
 def load():
     """Load libcfmi.so; raises if it has not been built."""
     global _lib
@@ -160,6 +176,7 @@ def load():
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python -c 'import "
             "__graft_entry__ as g; g.build()'` (make -C cajitafluids_amd/csrc)")
+    _torch_runtime_first()
     L = C.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(L, name)
