@@ -8,7 +8,9 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcfmi.so")
+# CFMI_LIBRARY names another build of the same library (A/B timing of
+# kernel variants); the default is the in-tree build.
+LIB_PATH = os.environ.get("CFMI_LIBRARY") or os.path.join(HERE, "libcfmi.so")
 
 CF_OK, CF_EINVAL, CF_ENOMEM, CF_ENOCONV, CF_EDEVICE, CF_ENOTIMPL = 0, 1, 2, 3, 4, 5
 STATUS_NAMES = {0: "CF_OK", 1: "CF_EINVAL", 2: "CF_ENOMEM", 3: "CF_ENOCONV",

@@ -540,10 +540,10 @@ void launch_adj_gradT(const Geo& g, Fields bar, double* y, double* ysum, RedBuf 
                       hipStream_t s)
 {
     if (g.dim == 2)
-        hipLaunchKernelGGL(k_adj_gradT<2>, dim3(kRedBlocks), dim3(kBlock), 0, s, g,
+        hipLaunchKernelGGL(k_adj_gradT<2>, dim3(stream_blocks(g)), dim3(kBlock), 0, s, g,
                            bar, y, ysum, rb);
     else
-        hipLaunchKernelGGL(k_adj_gradT<3>, dim3(kRedBlocks), dim3(kBlock), 0, s, g,
+        hipLaunchKernelGGL(k_adj_gradT<3>, dim3(stream_blocks(g)), dim3(kBlock), 0, s, g,
                            bar, y, ysum, rb);
 }
 
@@ -551,10 +551,10 @@ void launch_adj_shift(const Geo& g, double* y, const double* ysum, double diag_s
                       hipStream_t s)
 {
     if (g.dim == 2)
-        hipLaunchKernelGGL(k_adj_shift<2>, dim3(kRedBlocks), dim3(kBlock), 0, s, g,
+        hipLaunchKernelGGL(k_adj_shift<2>, dim3(stream_blocks(g)), dim3(kBlock), 0, s, g,
                            y, ysum, diag_sum);
     else
-        hipLaunchKernelGGL(k_adj_shift<3>, dim3(kRedBlocks), dim3(kBlock), 0, s, g,
+        hipLaunchKernelGGL(k_adj_shift<3>, dim3(stream_blocks(g)), dim3(kBlock), 0, s, g,
                            y, ysum, diag_sum);
 }
 

@@ -194,10 +194,10 @@ __global__ __launch_bounds__(256) void k_divergence(Geo g, Fields cur,
 void launch_divergence(const Geo& g, Fields cur, double* rhs, hipStream_t s)
 {
     if (g.dim == 2)
-        hipLaunchKernelGGL(k_divergence<2>, dim3(kRedBlocks), dim3(kBlock),
+        hipLaunchKernelGGL(k_divergence<2>, dim3(stream_blocks(g)), dim3(kBlock),
                            0, s, g, cur, rhs);
     else
-        hipLaunchKernelGGL(k_divergence<3>, dim3(kRedBlocks), dim3(kBlock),
+        hipLaunchKernelGGL(k_divergence<3>, dim3(stream_blocks(g)), dim3(kBlock),
                            0, s, g, cur, rhs);
 }
 

@@ -18,16 +18,21 @@ struct PcgState
 };
 
 // Grid-reduction scratch of the streaming kernels: per-block double-double
-// partials (kRedBlocks * 2 values * 2 doubles) and the arrival counter,
-// which the last block resets to 0.
+// partials (up to reduction_blocks_max * 3 values * 2 doubles) and the
+// arrival counters -- one per group of blocks (blockIdx & 7) and one for the
+// groups, each on its own 256-byte line (kCounterSlots * kCounterStride
+// unsigned) -- which the last block resets to 0.
 struct RedBuf
 {
     double* partials;
     unsigned* counter;
 };
 
+constexpr int kCounterStride = 64; // unsigned per counter line
+constexpr int kCounterSlots = 9;   // 8 group counters + the group counter
+
 constexpr int kBlock = 256;
-constexpr int kRedBlocks = 2048; // grid of the streaming kernels (multiple of 8)
+constexpr int kRedBlocks = 2048; // largest grid of the streaming kernels (multiple of 8)
 
 // State arrays of one version: q, u, v, w (w null in 2-D).
 struct Fields

@@ -456,7 +456,8 @@ __global__ __launch_bounds__(256) void k_rhs_shift(Geo g, double* rhs,
 
 // ---- launchers -------------------------------------------------------------------
 #define CF_STREAM_LAUNCH(kernel, ...)                                         \
-    hipLaunchKernelGGL(kernel, dim3(kRedBlocks), dim3(kBlock), 0, s, __VA_ARGS__)
+    hipLaunchKernelGGL(kernel, dim3(stream_blocks(g)), dim3(kBlock), 0, s,    \
+                       __VA_ARGS__)
 
 void launch_rhs_sum(const Geo& g, const double* rhs, double* out_sum,
                     RedBuf rb, hipStream_t s)

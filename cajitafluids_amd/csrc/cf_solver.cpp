@@ -181,7 +181,7 @@ VelocityCorrector::VelocityCorrector(ProblemManager& pm, const cf_config& cfg,
     _p[0].alloc(n, s);
     _p[1].alloc(n, s);
     _red_partials.alloc((int64_t)reduction_blocks_max(_g) * 3 * 2, s);
-    _red_counter.alloc(1, s);
+    _red_counter.alloc(kCounterSlots * kCounterStride, s);
     _scratch.alloc(8, s);
     void* p = nullptr;
     CF_HIP(hipMalloc(&p, sizeof(PcgState)));

@@ -28,8 +28,9 @@ namespace cfmi
 // Each block reduces its lanes' double-double sums (xor shuffles, then LDS),
 // and thread 0 publishes the block's partial with agent-scope atomic stores
 // (written through to memory, so every XCD sees them), waits for them to
-// complete, and counts its arrival with an agent-scope atomic.  The block
-// that arrives last reads all partials with agent-scope atomic loads (which
+// complete, and counts its arrival with agent-scope atomics (per group of
+// blocks, then per group).  The block that arrives last reads all partials
+// with agent-scope atomic loads (which
 // read memory, not its own XCD's L2), reduces them and runs `fin` on thread
 // 0.  No fence is issued: an agent-scope fence would write back and
 // invalidate the XCD's whole L2 on every block.
@@ -84,9 +85,25 @@ __device__ __forceinline__ void grid_reduce(double (&hi)[NV], double (&lo)[NV],
                                __HIP_MEMORY_SCOPE_AGENT);
         }
         __builtin_amdgcn_s_waitcnt(0); // partials complete before arrival
+        // Arrivals are counted per group of blocks (blockIdx & 7), each group
+        // on its own counter line: same-address atomics serialise at ~11 ns
+        // each, so one counter for a 1024-2048-block grid costs 12-24 us.
+        // The last block of a group then counts the group's arrival; the
+        // block that completes the last group reduces.
+        const unsigned groups = gridDim.x < 8 ? gridDim.x : 8;
+        const unsigned grp = blockIdx.x & 7;
+        const unsigned in_grp = (gridDim.x - 1 - grp) / 8 + 1;
         const unsigned t = __hip_atomic_fetch_add(
-            rb.counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        last = (t == gridDim.x - 1);
+            rb.counter + grp * kCounterStride, 1u, __ATOMIC_RELAXED,
+            __HIP_MEMORY_SCOPE_AGENT);
+        int fin_block = 0;
+        if (t == in_grp - 1) {
+            const unsigned u = __hip_atomic_fetch_add(
+                rb.counter + 8 * kCounterStride, 1u, __ATOMIC_RELAXED,
+                __HIP_MEMORY_SCOPE_AGENT);
+            fin_block = (u == groups - 1);
+        }
+        last = fin_block;
     }
     __syncthreads();
     if (!last)
@@ -115,9 +132,10 @@ __device__ __forceinline__ void grid_reduce(double (&hi)[NV], double (&lo)[NV],
         for (int v = 0; v < NV; ++v)
             sum[v] = h2[v] + l2[v];
         fin(sum);
-        __hip_atomic_store(rb.counter, 0u, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
     }
+    if (threadIdx.x < kCounterSlots)
+        __hip_atomic_store(rb.counter + threadIdx.x * kCounterStride, 0u,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---- row iteration --------------------------------------------------------------
@@ -226,6 +244,15 @@ __device__ __forceinline__ void for_owned_rows(const Geo& g, F&& f)
         const int k = r / n1;
         f(g.idx(g.h, g.h + j, k0 + k), j, k);
     }
+}
+
+// Grid of the row-streaming kernels: one block per owned row, rounded up to
+// a multiple of 8, and at most kRedBlocks (further rows are strided).
+inline int stream_blocks(const Geo& g)
+{
+    const int64_t rows = (int64_t)g.n[1] * g.n[2];
+    const int64_t b = (rows + 7) / 8 * 8;
+    return (int)(b < kRedBlocks ? b : kRedBlocks);
 }
 
 // Calls f(c, i, j, k) for every owned cell of this block (global indices).
