@@ -199,22 +199,37 @@ constexpr bool exact_products()
     return std::is_same<T, float>::value;
 }
 
+// Cells per lane along x of the 3-D kernels: 2 for FP32 (8-byte accesses);
+// CF_CPL64 for FP64.
+#ifndef CF_CPL64
+#define CF_CPL64 1
+#endif
 template <class T>
+constexpr int cpl_of()
+{
+    return std::is_same<T, float>::value ? 2 : CF_CPL64;
+}
+
+template <class T, int CPL>
 struct DirEpiT
 {
     const Geo& g;
     T* __restrict__ p_new;
     double hi[1] = {0.0}, lo[1] = {0.0};
-    __device__ void prefetch(const Tile3Ctx&, const Plane3Meta&, int) {}
+    __device__ void prefetch(const Tile3Ctx<CPL>&, const Plane3Meta&, int) {}
     __device__ void rotate() {}
-    __device__ void cell(const Tile3Ctx& c, const Plane3Meta& m, int q, T pc, T y)
+    __device__ void cells(const Tile3Ctx<CPL>& c, const Plane3Meta& m, int q,
+                          const T (&pc)[CPL], const T (&y)[CPL])
     {
-        p_new[c.colb[q] + m.off] = pc;
-        dd_acc_prod<exact_products<T>()>(hi[0], lo[0], (double)pc, (double)y);
+        vstore_valid<T, CPL>(p_new + c.colb[q] + m.off, pc, c.valid[q]);
+#pragma unroll
+        for (int j = 0; j < CPL; ++j)
+            if (c.valid[q][j])
+                dd_acc_prod<exact_products<T>()>(hi[0], lo[0], (double)pc[j], (double)y[j]);
     }
 };
 
-template <class T>
+template <class T, int CPL>
 struct UpdEpiT
 {
     const Geo& g;
@@ -222,38 +237,59 @@ struct UpdEpiT
     T* __restrict__ r;
     T alpha;
     double hi[2] = {0.0, 0.0}, lo[2] = {0.0, 0.0};
-    T xc[tile3::RY], rc[tile3::RY], xn[tile3::RY], rn_[tile3::RY];
-    __device__ void prefetch(const Tile3Ctx& c, const Plane3Meta& m, int k)
+    T xc[tile3::RY][CPL], rc[tile3::RY][CPL], xn[tile3::RY][CPL], rn_[tile3::RY][CPL];
+    __device__ void prefetch(const Tile3Ctx<CPL>& c, const Plane3Meta& m, int k)
     {
 #pragma unroll
         for (int q = 0; q < tile3::RY; ++q) {
-            const bool ok = k < c.k1 && c.valid[q];
+            // a group with a valid first cell lies inside its padded row
+            const bool ok = k < c.k1 && c.valid[q][0];
             const int64_t at = c.colb[q] + m.off;
-            xn[q] = ok ? x[at] : (T)0;
-            rn_[q] = ok ? r[at] : (T)0;
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) {
+                xn[q][j] = (T)0;
+                rn_[q][j] = (T)0;
+            }
+            if (ok) {
+                vload<T, CPL>(x + at, xn[q]);
+                vload<T, CPL>(r + at, rn_[q]);
+            }
         }
     }
     __device__ void rotate()
     {
 #pragma unroll
-        for (int q = 0; q < tile3::RY; ++q) {
-            xc[q] = xn[q];
-            rc[q] = rn_[q];
-        }
+        for (int q = 0; q < tile3::RY; ++q)
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) {
+                xc[q][j] = xn[q][j];
+                rc[q][j] = rn_[q][j];
+            }
     }
-    __device__ void cell(const Tile3Ctx& c, const Plane3Meta& m, int q, T pc, T qv)
+    __device__ void cells(const Tile3Ctx<CPL>& c, const Plane3Meta& m, int q,
+                          const T (&pc)[CPL], const T (&qv)[CPL])
     {
         const int64_t at = c.colb[q] + m.off;
-        x[at] = fma_t<T>(alpha, pc, xc[q]);
-        const T rn = fma_t<T>(-alpha, qv, rc[q]);
-        r[at] = rn;
-        const T z = dinv_pick<T>(g, c.wy[q] + m.wz, c.wx) * rn;
-        dd_acc_prod<exact_products<T>()>(hi[0], lo[0], (double)rn, (double)rn);
-        dd_acc_prod<exact_products<T>()>(hi[1], lo[1], (double)rn, (double)z);
+        T xo[CPL], ro[CPL];
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            xo[j] = fma_t<T>(alpha, pc[j], xc[q][j]);
+            ro[j] = fma_t<T>(-alpha, qv[j], rc[q][j]);
+        }
+        vstore_valid<T, CPL>(x + at, xo, c.valid[q]);
+        vstore_valid<T, CPL>(r + at, ro, c.valid[q]);
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            if (!c.valid[q][j])
+                continue;
+            const T z = dinv_pick<T>(g, c.wy[q] + m.wz, c.wx[j]) * ro[j];
+            dd_acc_prod<exact_products<T>()>(hi[0], lo[0], (double)ro[j], (double)ro[j]);
+            dd_acc_prod<exact_products<T>()>(hi[1], lo[1], (double)ro[j], (double)z);
+        }
     }
 };
 
-template <class T>
+template <class T, int CPL>
 __global__ __launch_bounds__(256) void k_pcg_dir_spmv3(
     Geo g, Tile3Grid tg, const T* __restrict__ r, const T* __restrict__ p_old,
     T* __restrict__ p_new, PcgState* st, RedBuf rb)
@@ -261,17 +297,17 @@ __global__ __launch_bounds__(256) void k_pcg_dir_spmv3(
     using namespace tile3;
     if (st->done)
         return;
-    __shared__ T pl[2][TY + 2][TX + 2];
+    __shared__ T pl[2][TY + 2][tx<CPL>() + 2];
     const bool first = st->iter == 0;
     const T beta = (T)st->beta;
-    DirEpiT<T> epi{g, p_new};
-    tile3_march<T, true>(g, tg, r, p_old, first, beta, pl, epi);
+    DirEpiT<T, CPL> epi{g, p_new};
+    tile3_march<T, true, CPL>(g, tg, r, p_old, first, beta, pl, epi);
     dd_norm(epi.hi[0], epi.lo[0]);
     grid_reduce<1>(epi.hi, epi.lo, rb,
                    [&](double* sm) { st->alpha = st->rho / sm[0]; });
 }
 
-template <class T>
+template <class T, int CPL>
 __global__ __launch_bounds__(256) void k_pcg_update3(
     Geo g, Tile3Grid tg, T* __restrict__ x, T* __restrict__ r,
     const T* __restrict__ p, PcgState* st, RedBuf rb)
@@ -279,9 +315,9 @@ __global__ __launch_bounds__(256) void k_pcg_update3(
     using namespace tile3;
     if (st->done)
         return;
-    __shared__ T pl[2][TY + 2][TX + 2];
-    UpdEpiT<T> epi{g, x, r, (T)st->alpha};
-    tile3_march<T, false>(g, tg, p, nullptr, false, (T)0, pl, epi);
+    __shared__ T pl[2][TY + 2][tx<CPL>() + 2];
+    UpdEpiT<T, CPL> epi{g, x, r, (T)st->alpha};
+    tile3_march<T, false, CPL>(g, tg, p, nullptr, false, (T)0, pl, epi);
     dd_norm(epi.hi[0], epi.lo[0]);
     dd_norm(epi.hi[1], epi.lo[1]);
     grid_reduce<2>(epi.hi, epi.lo, rb, [&](double* sm) {
@@ -514,9 +550,9 @@ void launch_pcg_dir_spmv(const Geo& g, const T* r, const T* p_old, T* p_new,
     if (g.dim == 2)
         CF_STREAM_LAUNCH((k_pcg_dir_spmv<2, T>), g, r, p_old, p_new, st, rb);
     else
-        hipLaunchKernelGGL(k_pcg_dir_spmv3<T>, dim3(tile3_blocks(g)),
-                           dim3(kBlock), 0, s, g, tile3_grid(g), r, p_old,
-                           p_new, st, rb);
+        hipLaunchKernelGGL((k_pcg_dir_spmv3<T, cpl_of<T>()>),
+                           dim3(tile3_blocks<cpl_of<T>()>(g)), dim3(kBlock), 0, s, g,
+                           tile3_grid<cpl_of<T>()>(g), r, p_old, p_new, st, rb);
 }
 
 template <class T>
@@ -526,13 +562,14 @@ void launch_pcg_update(const Geo& g, T* x, T* r, const T* p, PcgState* st,
     if (g.dim == 2)
         CF_STREAM_LAUNCH((k_pcg_update<2, T>), g, x, r, p, st, rb);
     else
-        hipLaunchKernelGGL(k_pcg_update3<T>, dim3(tile3_blocks(g)), dim3(kBlock),
-                           0, s, g, tile3_grid(g), x, r, p, st, rb);
+        hipLaunchKernelGGL((k_pcg_update3<T, cpl_of<T>()>),
+                           dim3(tile3_blocks<cpl_of<T>()>(g)), dim3(kBlock), 0, s, g,
+                           tile3_grid<cpl_of<T>()>(g), x, r, p, st, rb);
 }
 
 int reduction_blocks_max(const Geo& g)
 {
-    const int t = g.dim == 3 ? tile3_blocks(g) : 0;
+    const int t = g.dim == 3 ? tile3_blocks<1>(g) : 0; // the largest tile grid
     return t > kRedBlocks ? t : kRedBlocks;
 }
 

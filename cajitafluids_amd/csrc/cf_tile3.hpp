@@ -1,15 +1,18 @@
 // cf_tile3.hpp -- the 3-D PCG kernels: z-marching tiles with a software
 // pipeline over planes.
 //
-// A block owns a 64 x 16 (x, y) tile and marches up a chunk of z planes.
-// Each lane keeps its column's values at planes k-1, k, k+1 in registers (4
-// rows per lane); the tile's plane-k values plus a one-cell x/y halo ring sit
-// in LDS for the x and y neighbours.  Loads run two planes ahead of the
-// stencil: while plane k is computed, plane k+1 (loaded one step earlier) is
-// finished into LDS and plane k+2 is in flight, so a wave seldom waits on
-// memory latency.  Every vector element is read once per chunk; halo cells
-// come from the neighbouring tiles, which the XCD-ordered block mapping keeps
-// in the same L2.
+// A block owns a (64 CPL) x 16 (x, y) tile and marches up a chunk of z
+// planes.  Each lane owns CPL consecutive x cells in each of 4 rows (CPL = 2:
+// one 8-byte (FP32) or 16-byte (FP64) access per row, half the address and
+// memory instructions per cell) and keeps its cells' values at planes k-1,
+// k, k+1 in registers; the tile's plane-k values plus a one-cell x/y halo
+// ring sit in LDS for the x and y neighbours (a cell's x neighbour inside
+// the lane's own group is read from registers).  Loads run two planes ahead
+// of the stencil: while plane k is computed, plane k+1 (loaded one step
+// earlier) is finished into LDS and plane k+2 is in flight, so a wave seldom
+// waits on memory latency.  Every vector element is read once per chunk;
+// halo cells come from the neighbouring tiles, which the XCD-ordered block
+// mapping keeps in the same L2.
 //
 // Everything that depends on the lane's x position is computed once per
 // block, everything that depends on a row once per block and wave (rows are
@@ -23,13 +26,20 @@ namespace cfmi
 {
 namespace tile3
 {
-constexpr int TX = 64;          // lanes along x
-constexpr int NW = kBlock / TX; // waves per block
+constexpr int LX = 64;          // lanes along x
+constexpr int NW = kBlock / LX; // waves per block
 constexpr int RY = 4;           // rows per lane
 constexpr int TY = NW * RY;     // tile rows
-constexpr int NHALO = 2 * TX + 2 * TY;
-constexpr int NS = RY + 1;      // load slots per lane: RY own cells + 1 halo
-static_assert(NHALO <= kBlock, "one halo cell per lane at most");
+constexpr int NS = RY + 1;      // load slots per lane: RY own rows + 1 halo
+// halo slots: the two y-halo rows (one lane per CPL cells) and the two
+// x-halo columns (one lane per cell)
+constexpr int NHALO = 2 * LX + 2 * TY;
+static_assert(NHALO <= kBlock, "one halo slot per lane at most");
+template <int CPL>
+constexpr int tx()
+{
+    return LX * CPL; // tile cells along x
+}
 } // namespace tile3
 
 struct Tile3Grid
@@ -37,21 +47,25 @@ struct Tile3Grid
     int tiles_x, tiles_y, nchunk, kz, work;
 };
 
+template <int CPL>
 inline Tile3Grid tile3_grid(const Geo& g)
 {
     using namespace tile3;
     Tile3Grid t;
-    t.tiles_x = (g.n[0] + TX - 1) / TX;
+    t.tiles_x = (g.n[0] + tx<CPL>() - 1) / tx<CPL>();
     t.tiles_y = (g.n[1] + TY - 1) / TY;
-    t.kz = g.n[2] / 16 > 16 ? g.n[2] / 16 : 16;
+    // z chunks: as many blocks as the one-cell-per-lane tiling
+    const int zc = 16 * CPL;
+    t.kz = g.n[2] / zc > 16 / CPL ? g.n[2] / zc : 16 / CPL;
     t.nchunk = (g.n[2] + t.kz - 1) / t.kz;
     t.work = t.tiles_x * t.tiles_y * t.nchunk;
     return t;
 }
 
+template <int CPL>
 inline int tile3_blocks(const Geo& g)
 {
-    const Tile3Grid t = tile3_grid(g);
+    const Tile3Grid t = tile3_grid<CPL>(g);
     return (t.work + 7) / 8 * 8;
 }
 
@@ -90,82 +104,187 @@ __device__ __forceinline__ bool wall_hi(const Geo& g, int d, int gi)
     return !g.periodic[d] && gi > g.n[d] - 2;
 }
 
+// CPL consecutive elements as one vector access (8 or 16 bytes for CPL = 2;
+// the element index of a lane's first cell is even, so it is aligned).
+template <class T, int CPL>
+struct Vec
+{
+};
+template <class T>
+struct Vec<T, 1>
+{
+    using type = T;
+};
+template <>
+struct Vec<float, 2>
+{
+    using type = float2;
+};
+template <>
+struct Vec<double, 2>
+{
+    using type = double2;
+};
+
+template <class T, int CPL>
+__device__ __forceinline__ void vload(const T* __restrict__ p, T (&v)[CPL])
+{
+    if constexpr (CPL == 1) {
+        v[0] = *p;
+    } else {
+        using V = typename Vec<T, 2>::type;
+        const V w = *reinterpret_cast<const V*>(p);
+        v[0] = w.x;
+        v[1] = w.y;
+    }
+}
+
+template <class T, int CPL>
+__device__ __forceinline__ void vstore(T* __restrict__ p, const T (&v)[CPL])
+{
+    if constexpr (CPL == 1) {
+        *p = v[0];
+    } else {
+        using V = typename Vec<T, 2>::type;
+        V w;
+        w.x = v[0];
+        w.y = v[1];
+        *reinterpret_cast<V*>(p) = w;
+    }
+}
+
+// Stores the valid cells of a lane's group (all of them as one access when
+// every cell is valid; only the last tile of a row has partial groups).
+template <class T, int CPL>
+__device__ __forceinline__ void vstore_valid(T* __restrict__ p, const T (&v)[CPL],
+                                             const bool (&ok)[CPL])
+{
+    if constexpr (CPL == 1) {
+        if (ok[0])
+            *p = v[0];
+    } else {
+        if (ok[0] && ok[1])
+            vstore<T, 2>(p, v);
+        else if (ok[0])
+            p[0] = v[0];
+    }
+}
+
 // The block's place in the grid and its lanes' load slots.  Slot q < RY is
-// the lane's own cell in row w*RY + q; slot RY is the lane's halo cell (if
-// any).  colb is the linear index of the slot's (x, y) column at local
-// z = 0, or -1 for a zero ghost.
+// the lane's own CPL cells in row w*RY + q; slot RY is the lane's halo slot:
+// CPL cells of a y-halo row (hkind 1) or one cell of an x-halo column
+// (hkind 2).  colb is the linear index of the slot's first (x, y) column at
+// local z = 0, or -1 for a zero ghost.
+template <int CPL>
 struct Tile3Ctx
 {
     bool active;
     int i0, j0, k0, k1;
     int lane, w;
     int64_t colb[tile3::NS];
-    int hwxy;           // x + y wall count of the halo slot
-    int hrow, hcol;     // LDS position of the halo slot (hrow < 0: none)
+    // a group whose last cell is the row's end, on a periodic x axis: its
+    // second cell (one past the end) is the row's first cell (CPL = 2)
+    bool split, hsplit;
+    int wrapback;         // n0 - 1
+    int hkind;            // 0: no halo slot, 1: y-halo cells, 2: x-halo cell
+    int hrow, hcol;       // LDS position of the halo slot's first cell
+    int hwxy[CPL];        // x + y wall count of the halo slot's cells
     // own cells
-    int wx;             // x wall count of the lane's column
-    bool xwall_m, xwall_p;
-    int wy[tile3::RY];  // y wall count of each row (wave-uniform)
+    int wx[CPL];          // x wall count of the lane's columns
+    bool xwall_m[CPL], xwall_p[CPL];
+    int wy[tile3::RY];    // y wall count of each row (wave-uniform)
     bool ywall_m[tile3::RY], ywall_p[tile3::RY];
-    bool valid[tile3::RY];
+    bool valid[tile3::RY][CPL];
 };
 
-__device__ __forceinline__ Tile3Ctx tile3_ctx(const Geo& g, const Tile3Grid& t)
+template <int CPL>
+__device__ __forceinline__ Tile3Ctx<CPL> tile3_ctx(const Geo& g, const Tile3Grid& t)
 {
     using namespace tile3;
-    Tile3Ctx c;
+    Tile3Ctx<CPL> c;
     const int L = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
     c.active = L < t.work;
-    const int tx = L % t.tiles_x;
+    const int tx_ = L % t.tiles_x;
     const int rest = L / t.tiles_x;
     const int ty = rest % t.tiles_y;
     const int ch = rest / t.tiles_y;
-    c.i0 = tx * TX;
+    c.i0 = tx_ * tx<CPL>();
     c.j0 = ty * TY;
     c.k0 = ch * t.kz;
     c.k1 = min(c.k0 + t.kz, g.n[2]);
-    c.lane = threadIdx.x & (TX - 1);
-    c.w = __builtin_amdgcn_readfirstlane(threadIdx.x / TX);
+    c.lane = threadIdx.x & (LX - 1);
+    c.w = __builtin_amdgcn_readfirstlane(threadIdx.x / LX);
     auto col = [&](int gi, int gj) -> int64_t {
         const int a = wrap_or_ghost(g, 0, gi), b = wrap_or_ghost(g, 1, gj);
         if (a < 0 || b < 0)
             return -1;
         return g.sx + (g.h + a) + g.px * (int64_t)(g.h + b);
     };
-    const int gi = c.i0 + c.lane;
-    const int a = wrap_or_ghost(g, 0, gi);
-    c.wx = a < 0 ? 0 : wall_count(g, 0, a);
-    c.xwall_m = wall_lo(g, 0, gi);
-    c.xwall_p = wall_hi(g, 0, gi);
+    const int gi0 = c.i0 + c.lane * CPL;
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) {
+        const int gi = gi0 + j;
+        const int a = wrap_or_ghost(g, 0, gi);
+        c.wx[j] = a < 0 ? 0 : wall_count(g, 0, a);
+        c.xwall_m[j] = wall_lo(g, 0, gi);
+        c.xwall_p[j] = wall_hi(g, 0, gi);
+    }
 #pragma unroll
     for (int q = 0; q < RY; ++q) {
         const int gj = c.j0 + c.w * RY + q;
         const int b = wrap_or_ghost(g, 1, gj);
-        c.colb[q] = col(gi, gj);
+        // cells just past the domain stand in for their periodic images
+        // (or zero ghosts) as neighbours of the last own cells
+        c.colb[q] = col(gi0, gj);
         c.wy[q] = b < 0 ? 0 : wall_count(g, 1, b);
         c.ywall_m[q] = wall_lo(g, 1, gj);
         c.ywall_p[q] = wall_hi(g, 1, gj);
-        c.valid[q] = gi < g.n[0] && gj < g.n[1];
+#pragma unroll
+        for (int j = 0; j < CPL; ++j)
+            c.valid[q][j] = gi0 + j < g.n[0] && gj < g.n[1];
     }
+    c.split = CPL > 1 && g.periodic[0] && gi0 + 1 == g.n[0];
+    c.wrapback = g.n[0] - 1;
+    // halo slot
     const int hh = threadIdx.x;
-    int gih = 0, gjh = 0;
+    c.hsplit = false;
+    c.hkind = 0;
     c.hrow = -1;
     c.hcol = 0;
-    if (hh < TX) {
-        c.hrow = 0; c.hcol = hh + 1; gih = c.i0 + hh; gjh = c.j0 - 1;
-    } else if (hh < 2 * TX) {
-        c.hrow = TY + 1; c.hcol = hh - TX + 1; gih = c.i0 + hh - TX; gjh = c.j0 + TY;
-    } else if (hh < 2 * TX + TY) {
-        c.hrow = hh - 2 * TX + 1; c.hcol = 0; gih = c.i0 - 1; gjh = c.j0 + hh - 2 * TX;
+    c.colb[RY] = -1;
+#pragma unroll
+    for (int j = 0; j < CPL; ++j)
+        c.hwxy[j] = 0;
+    if (hh < 2 * LX) {
+        // y-halo rows: the CPL cells above or below this lane's column group
+        const int l2 = hh < LX ? hh : hh - LX;
+        const int gj = hh < LX ? c.j0 - 1 : c.j0 + TY;
+        const int gi = c.i0 + l2 * CPL;
+        c.hkind = 1;
+        c.hrow = hh < LX ? 0 : TY + 1;
+        c.hcol = l2 * CPL + 1;
+        c.colb[RY] = col(gi, gj);
+        c.hsplit = CPL > 1 && g.periodic[0] && gi + 1 == g.n[0];
+        const int b = wrap_or_ghost(g, 1, gj);
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            const int a = wrap_or_ghost(g, 0, gi + j);
+            c.hwxy[j] = (a < 0 ? 0 : wall_count(g, 0, a)) + (b < 0 ? 0 : wall_count(g, 1, b));
+        }
     } else if (hh < NHALO) {
-        c.hrow = hh - 2 * TX - TY + 1; c.hcol = TX + 1; gih = c.i0 + TX;
-        gjh = c.j0 + hh - 2 * TX - TY;
+        // x-halo columns: one cell left or right of the tile
+        const int left = hh < 2 * LX + TY;
+        const int r = left ? hh - 2 * LX : hh - 2 * LX - TY;
+        const int gi = left ? c.i0 - 1 : c.i0 + tx<CPL>();
+        const int gj = c.j0 + r;
+        c.hkind = 2;
+        c.hrow = r + 1;
+        c.hcol = left ? 0 : tx<CPL>() + 1;
+        c.colb[RY] = col(gi, gj);
+        if (c.colb[RY] >= 0)
+            c.hwxy[0] = wall_count(g, 0, wrap_or_ghost(g, 0, gi)) +
+                        wall_count(g, 1, wrap_or_ghost(g, 1, gj));
     }
-    c.colb[RY] = c.hrow >= 0 ? col(gih, gjh) : -1;
-    c.hwxy = 0;
-    if (c.colb[RY] >= 0)
-        c.hwxy = wall_count(g, 0, wrap_or_ghost(g, 0, gih)) +
-                 wall_count(g, 1, wrap_or_ghost(g, 1, gjh));
     return c;
 }
 
@@ -190,85 +309,128 @@ __device__ __forceinline__ Plane3Meta plane3_meta(const Geo& g, int gk)
 }
 
 // Raw operands of one plane: a = r (direction) or p (update), b = old
-// direction (direction kernel only), both 0 at zero ghosts.
-template <class T, bool DIR>
+// direction (direction kernel only), all 0 at zero ghosts.
+template <class T, bool DIR, int CPL>
 struct Plane3
 {
-    T a[tile3::NS];
-    T b[tile3::NS];
+    T a[tile3::NS][CPL];
+    T b[tile3::NS][CPL];
 };
 
-template <class T, bool DIR>
-__device__ __forceinline__ void tile3_load(const Tile3Ctx& c, const Plane3Meta& m,
+template <class T, bool DIR, int CPL>
+__device__ __forceinline__ void tile3_load(const Tile3Ctx<CPL>& c, const Plane3Meta& m,
                                            const T* __restrict__ va,
                                            const T* __restrict__ vb, bool need_b,
-                                           Plane3<T, DIR>& P)
+                                           Plane3<T, DIR, CPL>& P)
 {
 #pragma unroll
     for (int s = 0; s < tile3::NS; ++s) {
         const bool ok = c.colb[s] >= 0 && m.off >= 0;
         const int64_t at = c.colb[s] + m.off;
-        P.a[s] = ok ? va[at] : (T)0;
-        if constexpr (DIR)
-            P.b[s] = (ok && need_b) ? vb[at] : (T)0;
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            P.a[s][j] = (T)0;
+            if constexpr (DIR)
+                P.b[s][j] = (T)0;
+        }
+        // the halo slot of an x-halo lane is one cell; every other slot is a
+        // lane's group of CPL cells, read as one access unless it wraps
+        const bool single = CPL > 1 && s == tile3::RY && c.hkind == 2;
+        const bool split = CPL > 1 && (s < tile3::RY ? c.split : c.hsplit);
+        if (ok && !single && !split) {
+            vload<T, CPL>(va + at, P.a[s]);
+            if constexpr (DIR)
+                if (need_b)
+                    vload<T, CPL>(vb + at, P.b[s]);
+        } else if (ok) {
+            P.a[s][0] = va[at];
+            if constexpr (DIR)
+                if (need_b)
+                    P.b[s][0] = vb[at];
+            if (split) {
+                // the row's first cell: n0 - 1 cells back from the last
+                const int64_t at0 = at - c.wrapback;
+                P.a[s][CPL - 1] = va[at0];
+                if constexpr (DIR)
+                    if (need_b)
+                        P.b[s][CPL - 1] = vb[at0];
+            }
+        }
     }
 }
 
-// Value of slot s: the new direction fma(beta, p_old, D^-1 r) (D^-1 r
-// alone on the first iteration) for DIR, else the loaded p.
-template <class T, bool DIR>
-__device__ __forceinline__ T tile3_value(const Geo& g, const Tile3Ctx& c,
+// Value of cell j of slot s: the new direction fma(beta, p_old, D^-1 r)
+// (D^-1 r alone on the first iteration) for DIR, else the loaded p.
+template <class T, bool DIR, int CPL>
+__device__ __forceinline__ T tile3_value(const Geo& g, const Tile3Ctx<CPL>& c,
                                          const Plane3Meta& m,
-                                         const Plane3<T, DIR>& P, int s,
+                                         const Plane3<T, DIR, CPL>& P, int s, int j,
                                          bool first, T beta)
 {
     if constexpr (!DIR) {
-        return P.a[s];
+        return P.a[s][j];
     } else {
         T dinv;
         if (s < tile3::RY)
-            dinv = dinv_pick<T>(g, c.wy[s] + m.wz, c.wx); // uniform y, z part
+            dinv = dinv_pick<T>(g, c.wy[s] + m.wz, c.wx[j]); // uniform y, z part
         else
-            dinv = dinv_sel<T>(g, c.hwxy + m.wz);
-        const T z = dinv * P.a[s];
-        return first ? z : fma_t<T>(beta, P.b[s], z);
+            dinv = dinv_sel<T>(g, c.hwxy[j] + m.wz);
+        const T z = dinv * P.a[s][j];
+        return first ? z : fma_t<T>(beta, P.b[s][j], z);
     }
 }
 
 // Writes the plane's values into LDS plane pl and own[]; the halo slot
-// fills its ring cell.
-template <class T, bool DIR>
-__device__ __forceinline__ void tile3_finish(const Geo& g, const Tile3Ctx& c,
+// fills its ring cells.
+template <class T, bool DIR, int CPL>
+__device__ __forceinline__ void tile3_finish(const Geo& g, const Tile3Ctx<CPL>& c,
                                              const Plane3Meta& m,
-                                             const Plane3<T, DIR>& P, bool first,
-                                             T beta, T (*pl)[tile3::TX + 2],
-                                             T (&own)[tile3::RY])
+                                             const Plane3<T, DIR, CPL>& P, bool first,
+                                             T beta, T (*pl)[tile3::tx<CPL>() + 2],
+                                             T (&own)[tile3::RY][CPL])
 {
     using namespace tile3;
 #pragma unroll
     for (int q = 0; q < RY; ++q) {
-        own[q] = tile3_value<T, DIR>(g, c, m, P, q, first, beta);
-        pl[c.w * RY + q + 1][c.lane + 1] = own[q];
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            own[q][j] = tile3_value<T, DIR, CPL>(g, c, m, P, q, j, first, beta);
+            pl[c.w * RY + q + 1][c.lane * CPL + j + 1] = own[q][j];
+        }
     }
-    if (c.hrow >= 0)
-        pl[c.hrow][c.hcol] = tile3_value<T, DIR>(g, c, m, P, RY, first, beta);
+    if constexpr (CPL == 1) {
+        if (c.hkind != 0)
+            pl[c.hrow][c.hcol] = tile3_value<T, DIR, CPL>(g, c, m, P, RY, 0, first, beta);
+    } else {
+        if (c.hkind == 1) {
+#pragma unroll
+            for (int j = 0; j < CPL; ++j)
+                pl[c.hrow][c.hcol + j] =
+                    tile3_value<T, DIR, CPL>(g, c, m, P, RY, j, first, beta);
+        } else if (c.hkind == 2) {
+            pl[c.hrow][c.hcol] = tile3_value<T, DIR, CPL>(g, c, m, P, RY, 0, first, beta);
+        }
+    }
 }
 
-// y = A p at own cell (row q) of a plane: centre pc, z neighbours pm / pp,
-// x and y neighbours from the LDS plane; reference stencil order
-// {0}, -x, +x, -y, +y, -z, +z with zero couplings across walls.
-template <class T>
-__device__ __forceinline__ T tile3_apply(const Geo& g, const Tile3Ctx& c,
+// y = A p at own cell (row q, cell j) of a plane: centre pc[j], z neighbours
+// pm / pp, x neighbours from the lane's group or the LDS plane, y neighbours
+// from the LDS plane; reference stencil order {0}, -x, +x, -y, +y, -z, +z
+// with zero couplings across walls.
+template <class T, int CPL>
+__device__ __forceinline__ T tile3_apply(const Geo& g, const Tile3Ctx<CPL>& c,
                                          const Plane3Meta& m,
-                                         const T (*pl)[tile3::TX + 2], int q,
-                                         T pc, T pm, T pp)
+                                         const T (*pl)[tile3::tx<CPL>() + 2], int q,
+                                         int j, const T (&pc)[CPL], T pm, T pp)
 {
     const T s = offdiag<T>(g);
     const int jj = c.w * tile3::RY + q;
-    const int ln = c.lane;
-    T y = diag_pick<T>(g, c.wy[q] + m.wz, c.wx) * pc;
-    y = fma_t<T>(c.xwall_m ? (T)0 : s, pl[jj + 1][ln], y);
-    y = fma_t<T>(c.xwall_p ? (T)0 : s, pl[jj + 1][ln + 2], y);
+    const int ln = c.lane * CPL + j; // LDS column of the cell is ln + 1
+    const T xm = j > 0 ? pc[j > 0 ? j - 1 : 0] : pl[jj + 1][ln];
+    const T xp = j < CPL - 1 ? pc[j < CPL - 1 ? j + 1 : 0] : pl[jj + 1][ln + 2];
+    T y = diag_pick<T>(g, c.wy[q] + m.wz, c.wx[j]) * pc[j];
+    y = fma_t<T>(c.xwall_m[j] ? (T)0 : s, xm, y);
+    y = fma_t<T>(c.xwall_p[j] ? (T)0 : s, xp, y);
     y = fma_t<T>(c.ywall_m[q] ? (T)0 : s, pl[jj][ln + 1], y);
     y = fma_t<T>(c.ywall_p[q] ? (T)0 : s, pl[jj + 2][ln + 1], y);
     y = fma_t<T>(m.zwall_m ? (T)0 : s, pm, y);
@@ -276,33 +438,34 @@ __device__ __forceinline__ T tile3_apply(const Geo& g, const Tile3Ctx& c,
     return y;
 }
 
-// The z march shared by the direction (DIR) and update kernels; epi.cell()
-// receives every owned cell's centre value and A p.
-template <class T, bool DIR, class Epi>
-__device__ __forceinline__ void tile3_march(const Geo& g, const Tile3Grid& tg,
-                                            const T* __restrict__ va,
-                                            const T* __restrict__ vb, bool first,
-                                            T beta, T (*pl)[tile3::TY + 2][tile3::TX + 2],
-                                            Epi&& epi)
+// The z march shared by the direction (DIR) and update kernels;
+// epi.cells() receives each own row's centre values and A p.
+template <class T, bool DIR, int CPL, class Epi>
+__device__ __forceinline__ void tile3_march(
+    const Geo& g, const Tile3Grid& tg, const T* __restrict__ va,
+    const T* __restrict__ vb, bool first, T beta,
+    T (*pl)[tile3::TY + 2][tile3::tx<CPL>() + 2], Epi&& epi)
 {
     using namespace tile3;
-    const Tile3Ctx c = tile3_ctx(g, tg);
+    const Tile3Ctx<CPL> c = tile3_ctx<CPL>(g, tg);
     if (!c.active)
         return;
     const bool need_b = !first;
-    Plane3<T, DIR> P0, P1;
-    T pm[RY], pc[RY], pp[RY];
+    Plane3<T, DIR, CPL> P0, P1;
+    T pm[RY][CPL], pc[RY][CPL], pp[RY][CPL];
     // prologue: plane k0-1 (own column only), plane k0 into LDS, k0+1 in flight
     Plane3Meta mm = plane3_meta(g, c.k0 - 1);
-    tile3_load<T, DIR>(c, mm, va, vb, need_b, P0);
+    tile3_load<T, DIR, CPL>(c, mm, va, vb, need_b, P0);
 #pragma unroll
     for (int q = 0; q < RY; ++q)
-        pm[q] = tile3_value<T, DIR>(g, c, mm, P0, q, first, beta);
+#pragma unroll
+        for (int j = 0; j < CPL; ++j)
+            pm[q][j] = tile3_value<T, DIR, CPL>(g, c, mm, P0, q, j, first, beta);
     Plane3Meta mc = plane3_meta(g, c.k0);
-    tile3_load<T, DIR>(c, mc, va, vb, need_b, P1);
-    tile3_finish<T, DIR>(g, c, mc, P1, first, beta, pl[0], pc);
+    tile3_load<T, DIR, CPL>(c, mc, va, vb, need_b, P1);
+    tile3_finish<T, DIR, CPL>(g, c, mc, P1, first, beta, pl[0], pc);
     Plane3Meta mn = plane3_meta(g, c.k0 + 1);
-    tile3_load<T, DIR>(c, mn, va, vb, need_b, P0);
+    tile3_load<T, DIR, CPL>(c, mn, va, vb, need_b, P0);
     epi.prefetch(c, mc, c.k0);
     epi.rotate();
     __syncthreads();
@@ -311,17 +474,25 @@ __device__ __forceinline__ void tile3_march(const Geo& g, const Tile3Grid& tg,
         // P0 holds plane k+1 (loaded last step); start plane k+2
         const Plane3Meta mnn = plane3_meta(g, k + 2);
         if (k + 2 <= c.k1)
-            tile3_load<T, DIR>(c, mnn, va, vb, need_b, P1);
-        tile3_finish<T, DIR>(g, c, mn, P0, first, beta, pl[cur ^ 1], pp);
+            tile3_load<T, DIR, CPL>(c, mnn, va, vb, need_b, P1);
+        tile3_finish<T, DIR, CPL>(g, c, mn, P0, first, beta, pl[cur ^ 1], pp);
         epi.prefetch(c, mn, k + 1);
 #pragma unroll
         for (int q = 0; q < RY; ++q) {
-            if (c.valid[q]) {
-                const T y = tile3_apply<T>(g, c, mc, pl[cur], q, pc[q], pm[q], pp[q]);
-                epi.cell(c, mc, q, pc[q], y);
+            // a group with no valid cell has nothing to compute or store
+            if (c.valid[q][0]) {
+                T y[CPL];
+#pragma unroll
+                for (int j = 0; j < CPL; ++j)
+                    y[j] = tile3_apply<T, CPL>(g, c, mc, pl[cur], q, j, pc[q], pm[q][j],
+                                               pp[q][j]);
+                epi.cells(c, mc, q, pc[q], y);
             }
-            pm[q] = pc[q];
-            pc[q] = pp[q];
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) {
+                pm[q][j] = pc[q][j];
+                pc[q][j] = pp[q][j];
+            }
         }
         __syncthreads();
         cur ^= 1;
