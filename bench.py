@@ -146,9 +146,9 @@ def main():
         dom = max(per, key=lambda k: per[k]["avg_us"])
         d = per[dom]
         achieved = d["bytes"] / (d["avg_us"] * 1e-6) / 1e9
-        kname = f"k_pcg_{dom}3<double>" if n == 256 else None
+        kname = f"k_pcg_{dom}3<double, 1>" if n == 256 else None
         traffic = measured_traffic(kname) if kname else None
-        roofline = {"bound": "hbm", "kernel": f"k_pcg_{dom}3<double>", "achieved": round(achieved, 1),
+        roofline = {"bound": "hbm", "kernel": f"k_pcg_{dom}3<double, 1>", "achieved": round(achieved, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "traffic_note": TRAFFIC_NOTE if traffic else None,

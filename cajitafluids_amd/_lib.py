@@ -6,6 +6,7 @@ path.
 """
 import ctypes as C
 import os
+import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 # CFMI_LIBRARY names another build of the same library (A/B timing of
@@ -157,10 +158,11 @@ def _torch_runtime_first():
     """The torch wheel carries its own HIP/HSA runtime next to the system one
     that libcfmi.so links.  Both can serve one process only if torch's
     initialises first: in the other order torch finds no device.  So when
-    torch is importable, its runtime is brought up before libcfmi.so loads."""
-    try:
-        import torch
-    except Exception:
+    the process has imported torch, its runtime is brought up before
+    libcfmi.so loads.  A process without torch keeps a single runtime (which
+    is what profilers that intercept the runtime need)."""
+    torch = sys.modules.get("torch")
+    if torch is None:
         return
     try:
         if torch.cuda.is_available():

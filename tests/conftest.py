@@ -10,6 +10,19 @@ for p in (REPO, TESTS):
         sys.path.insert(0, p)
 
 
+def _torch_first():
+    """Tests that hand torch tensors to the library need torch's HIP runtime
+    initialised before libcfmi.so's (the library does this when torch is
+    already imported)."""
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
+
+
+_torch_first()
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
     config.addinivalue_line("markers", "slow: long-running CPU test")
