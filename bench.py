@@ -49,6 +49,9 @@ def parse():
                          "box (10 steps, checkpoint every 2) against their forwards")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only the reverse-mode leg (profiling runs)")
+    ap.add_argument("--summarize-counters", metavar="CSV",
+                    help="print per-kernel dispatch count, median and sum of a "
+                         "rocprofv3 --pmc counter CSV, then exit (no GPU use)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a plain 1 GiB device copy, the reference "
                          "kernel for scaling rocprof FETCH_SIZE / WRITE_SIZE")
@@ -96,8 +99,27 @@ def timed_run(sol, steps, warmup, timing=True):
     return t, iters, outer, res, kt
 
 
+def summarize_counters(path):
+    """Per-kernel dispatch count, median and sum of each counter in a
+    rocprofv3 counter_collection CSV, largest sum first."""
+    import csv
+    import statistics
+    vals = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            vals.setdefault((r["Kernel_Name"], r["Counter_Name"]), []).append(
+                float(r["Counter_Value"]))
+    w = csv.writer(sys.stdout)
+    w.writerow(["kernel", "counter", "dispatches", "median", "sum"])
+    for (k, c), v in sorted(vals.items(), key=lambda kv: -sum(kv[1])):
+        w.writerow([k[:120], c, len(v), statistics.median(v), sum(v)])
+
+
 def main():
     args = parse()
+    if args.summarize_counters:
+        summarize_counters(args.summarize_counters)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the "
                           "multi-GPU design is described in DESIGN.md",
