@@ -39,7 +39,7 @@ def parse():
                     help="skip the FP64 leg (profiling runs only; no JSON line)")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--cpu-iters", type=int, default=400)
     ap.add_argument("--timing-sample", type=int, default=8,
                     help="time the PCG kernels with HIP events in one "
                          "iteration out of N inside the timed region (0: off)")
