@@ -52,6 +52,10 @@ class CfConfig(C.Structure):
         ("max_outer", C.c_int),
         ("quirks", C.c_int),
         ("refine_mode", C.c_int),
+        ("preconditioner", C.c_int),
+        ("mg_nu", C.c_int),
+        ("mg_omega", C.c_double),
+        ("mg_coarse_sweeps", C.c_int),
     ]
 
 

@@ -12,6 +12,7 @@ SOLID, FREE = 0, 1
 INFLOW_NONE, INFLOW_BOX, INFLOW_SPHERE = 0, 1, 2
 FP64, FP32_REFINED = 0, 1
 REFINE_RESTART, REFINE_CARRY = 0, 1
+PRECOND_JACOBI, PRECOND_MG = 0, 1
 QUIRK_RK3_V0 = 1      # TimeIntegrator.hpp:57-58 (3/4 probe uses v0)
 QUIRK_PROJ_BC_U = 2   # VelocityCorrector.hpp:260 (FaceJ BC applied to u)
 
@@ -46,6 +47,10 @@ def base_config(dim=2, n=128):
         max_outer=50,
         quirks=QUIRK_RK3_V0 | (QUIRK_PROJ_BC_U if dim == 2 else 0),
         refine_mode=REFINE_CARRY,
+        preconditioner=PRECOND_JACOBI,
+        mg_nu=2,
+        mg_omega=0.8,
+        mg_coarse_sweeps=16,
     )
 
 

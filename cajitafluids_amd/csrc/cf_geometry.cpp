@@ -52,6 +52,26 @@ cf_status build_geometry(const cf_config& c, Geo& g, const char** why)
         *why = "cg_tol and inner_tol must be positive, max_outer >= 0";
         return CF_EINVAL;
     }
+    if (c.preconditioner != CF_PRECOND_JACOBI && c.preconditioner != CF_PRECOND_MG) {
+        *why = "preconditioner must be CF_PRECOND_JACOBI or CF_PRECOND_MG";
+        return CF_EINVAL;
+    }
+    if (c.preconditioner == CF_PRECOND_MG) {
+        if (c.precision != CF_FP64) {
+            *why = "the multigrid preconditioner runs in FP64 only";
+            return CF_ENOTIMPL;
+        }
+        if (c.mg_nu < 1 || c.mg_coarse_sweeps < 1 || !(c.mg_omega > 0) ||
+            !(c.mg_omega <= 1)) {
+            *why = "mg_nu and mg_coarse_sweeps must be >= 1, mg_omega in (0, 1]";
+            return CF_EINVAL;
+        }
+        for (int d = 0; d < c.dim; ++d)
+            if (c.periodic[d] && c.num_cell[d] < 2) {
+                *why = "multigrid needs at least 2 cells along a periodic axis";
+                return CF_EINVAL;
+            }
+    }
     g.dim = c.dim;
     g.h = c.halo;
     for (int d = 0; d < 3; ++d) {

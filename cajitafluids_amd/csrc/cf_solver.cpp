@@ -439,6 +439,8 @@ cf_status VelocityCorrector::solve(int precision, const double* b, double* x,
         return solveRefined(b, x, st);
     if (precision != CF_FP64)
         throw CfError(CF_EINVAL, "precision must be CF_FP64 or CF_FP32_REFINED");
+    if (_cfg.preconditioner == CF_PRECOND_MG)
+        throw CfError(CF_ENOTIMPL, "the multigrid preconditioner is not built");
     return solve64(b, x, st);
 }
 

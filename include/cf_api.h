@@ -37,6 +37,7 @@ enum { CF_SOLID = 0, CF_FREE = 1 }; /* BoundaryType (BoundaryConditions.hpp:22-2
 enum { CF_INFLOW_NONE = 0, CF_INFLOW_BOX = 1, CF_INFLOW_SPHERE = 2 };
 enum { CF_FP64 = 0, CF_FP32_REFINED = 1 };
 enum { CF_REFINE_RESTART = 0, CF_REFINE_CARRY = 1 };
+enum { CF_PRECOND_JACOBI = 0, CF_PRECOND_MG = 1 };
 
 /* As-written reference behaviours, reproduced when set. */
 #define CF_QUIRK_RK3_V0 1    /* 3/4-step probe uses v0: TimeIntegrator.hpp:57-58 */
@@ -83,6 +84,14 @@ typedef struct cf_config {
                              PCG; CF_REFINE_CARRY: passes keep the search
                              direction (one FP32 PCG run whose residual is
                              replaced by the FP64 one at each pass) */
+    /* Pressure-solve preconditioner (VelocityCorrector.hpp:300-339 names
+     * the choice; the reference's default is HYPRE's PCG + PFMG): Jacobi,
+     * or a geometric multigrid V-cycle (FP64 only). */
+    int preconditioner;   /* CF_PRECOND_JACOBI or CF_PRECOND_MG */
+    int mg_nu;            /* Jacobi sweeps before and after each coarse
+                             correction (>= 1) */
+    double mg_omega;      /* Jacobi damping in (0, 1] */
+    int mg_coarse_sweeps; /* Jacobi sweeps on the coarsest level (>= 1) */
 } cf_config;
 
 typedef struct cf_solve_stats {

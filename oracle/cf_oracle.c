@@ -219,6 +219,7 @@ void cfo_destroy(cfo_state* s)
     free(s->z32);
     free(s->p32);
     free(s->q32);
+    cfo_mg_free(s);
     free(s);
 }
 
@@ -622,8 +623,7 @@ void cfo_apply_A(cfo_state* s, const double* x, double* y)
     cfo_spmv(s, x, y);
 }
 
-static double cfo_dot_dd(const struct cfo_state* s, const double* a,
-                         const double* b)
+double cfo_dot_dd(const struct cfo_state* s, const double* a, const double* b)
 {
     int lo[3], hi[3];
     cfo_owned_range(s, 0, lo, hi);
@@ -671,19 +671,6 @@ static double cfo_dot32(const struct cfo_state* s, const float* a,
     return cfo_dd_value(total);
 }
 
-/* Elementwise helper over owned cells: body sees the linear index c. */
-#define CFO_FOR_OWNED_CELLS(s, body)                                         \
-    do {                                                                     \
-        int lo_[3], hi_[3];                                                  \
-        cfo_owned_range((s), 0, lo_, hi_);                                   \
-        _Pragma("omp parallel for collapse(2) schedule(static)")             \
-        for (int k_ = lo_[2]; k_ < hi_[2]; ++k_)                             \
-            for (int j_ = lo_[1]; j_ < hi_[1]; ++j_)                         \
-                for (int i_ = lo_[0]; i_ < hi_[0]; ++i_) {                   \
-                    long c = cfo_idx((s), 0, i_, j_, k_);                    \
-                    body;                                                    \
-                }                                                            \
-    } while (0)
 
 /*
  * Jacobi-PCG, as this build defines Cajita's ReferenceConjugateGradient:
@@ -1021,7 +1008,15 @@ int cfo_pressure_solve(cfo_state* s, int precision, cfo_stats* st)
             return cfo_pcg_carry(s, s->rhs, s->lhs, st);
         return cfo_pcg_refined(s, s->rhs, s->lhs, st);
     }
-    return cfo_pcg64(s, s->rhs, s->lhs, s->c.cg_tol, s->c.cg_max_iter, st);
+    return cfo_solve64(s, s->rhs, s->lhs, s->c.cg_tol, s->c.cg_max_iter, st);
+}
+
+int cfo_solve64(struct cfo_state* s, const double* b, double* x, double tol,
+                int maxit, cfo_stats* st)
+{
+    if (s->c.preconditioner == CFO_PRECOND_MG)
+        return cfo_pcg_mg(s, b, x, tol, maxit, st);
+    return cfo_pcg64(s, b, x, tol, maxit, st);
 }
 
 /* VelocityCorrector.hpp:266-282.  A solve that does not converge aborts the

@@ -85,9 +85,14 @@ typedef struct cfo_config {
     int max_outer;
     int quirks;
     int refine_mode; /* CFO_REFINE_RESTART or CFO_REFINE_CARRY */
+    int preconditioner;   /* CFO_PRECOND_JACOBI or CFO_PRECOND_MG */
+    int mg_nu;            /* Jacobi sweeps before and after the coarse correction */
+    double mg_omega;      /* Jacobi damping */
+    int mg_coarse_sweeps; /* Jacobi sweeps on the coarsest level */
 } cfo_config;
 
 enum { CFO_REFINE_RESTART = 0, CFO_REFINE_CARRY = 1 };
+enum { CFO_PRECOND_JACOBI = 0, CFO_PRECOND_MG = 1 };
 
 typedef struct cfo_stats {
     int converged;
@@ -141,6 +146,14 @@ double cfo_rand_normal(uint64_t seed, uint64_t stream, uint64_t idx);
 void cfo_init_vortex_tubes(cfo_state* s, uint64_t seed, double peak);
 void cfo_init_blobs(cfo_state* s, double* out_cells, uint64_t seed, int nblobs);
 void cfo_init_random_faces(cfo_state* s, uint64_t seed);
+
+/* Multigrid preconditioner (cf_oracle_mg.c): level count, level cell
+ * counts, the level operator applied to a compact (x fastest, owned cells
+ * only) vector, and one V-cycle z = B r on compact level-0 vectors. */
+int cfo_mg_nlevels(cfo_state* s);
+int cfo_mg_level_dims(cfo_state* s, int level, int n[3]);
+int cfo_mg_apply_level(cfo_state* s, int level, const double* x, double* y);
+int cfo_mg_vcycle(cfo_state* s, const double* r, double* z);
 
 /* Adjoint (reverse mode) of setup + nsteps, checkpointing every ckpt steps.
  * Inputs: the initial state held in Current, the loss weights dL/dq_T in

@@ -380,7 +380,7 @@ static int project_adj(struct cfo_state* s, double** bar, double* y, double* z)
     }
     memset(z, 0, (size_t)s->size[0] * sizeof(double));
     cfo_stats st;
-    int rc = cfo_pcg64(s, y, z, s->c.cg_tol, s->c.cg_max_iter, &st);
+    int rc = cfo_solve64(s, y, z, s->c.cg_tol, s->c.cg_max_iter, &st);
     if (rc != CFO_OK)
         return rc;
     /* 4. bar += -(1/dx) D^T z, then the divergence gather's transpose */

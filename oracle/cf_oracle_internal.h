@@ -30,6 +30,7 @@ struct cfo_state {
     float *x32, *r32, *z32, *p32, *q32;
     double *A_scratch;
     int nent; /* dim + 1 entities */
+    struct cfo_mg* mg; /* multigrid hierarchy, built on first use */
 };
 
 /* Double-double accumulator: result rounding is independent of the order in
@@ -79,6 +80,26 @@ int cfo_in_source(const struct cfo_state* s, const double x[3]);
 void cfo_spmv(struct cfo_state* s, const double* x, double* y);
 int cfo_pcg64(struct cfo_state* s, const double* b, double* x, double tol,
               int maxit, cfo_stats* st);
+int cfo_pcg_mg(struct cfo_state* s, const double* b, double* x, double tol,
+               int maxit, cfo_stats* st);
+/* the FP64 solve of the configured preconditioner */
+int cfo_solve64(struct cfo_state* s, const double* b, double* x, double tol,
+                int maxit, cfo_stats* st);
+void cfo_mg_free(struct cfo_state* s);
+double cfo_dot_dd(const struct cfo_state* s, const double* a, const double* b);
+
+#define CFO_FOR_OWNED_CELLS(s, body)                                         \
+    do {                                                                     \
+        int lo_[3], hi_[3];                                                  \
+        cfo_owned_range((s), 0, lo_, hi_);                                   \
+        _Pragma("omp parallel for collapse(2) schedule(static)")             \
+        for (int k_ = lo_[2]; k_ < hi_[2]; ++k_)                             \
+            for (int j_ = lo_[1]; j_ < hi_[1]; ++j_)                         \
+                for (int i_ = lo_[0]; i_ < hi_[0]; ++i_) {                   \
+                    long c = cfo_idx((s), 0, i_, j_, k_);                    \
+                    body;                                                    \
+                }                                                            \
+    } while (0)
 void cfo_apply_pressure_arrays(struct cfo_state* s, double* u, double* v,
                                double* w, const double* p);
 void cfo_divergence(struct cfo_state* s, const double* u, const double* v,

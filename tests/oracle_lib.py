@@ -53,6 +53,10 @@ class CfoConfig(C.Structure):
         ("max_outer", C.c_int),
         ("quirks", C.c_int),
         ("refine_mode", C.c_int),
+        ("preconditioner", C.c_int),
+        ("mg_nu", C.c_int),
+        ("mg_omega", C.c_double),
+        ("mg_coarse_sweeps", C.c_int),
     ]
 
 
@@ -139,6 +143,10 @@ def lib():
         "cfo_init_blobs": (None, [P, dp, C.c_uint64, C.c_int]),
         "cfo_init_random_faces": (None, [P, C.c_uint64]),
         "cfo_adjoint_run": (C.c_int, [P, C.c_int, C.c_int, dp, dp, dp, dp, dp, dp]),
+        "cfo_mg_nlevels": (C.c_int, [P]),
+        "cfo_mg_level_dims": (C.c_int, [P, C.c_int, C.POINTER(C.c_int)]),
+        "cfo_mg_apply_level": (C.c_int, [P, C.c_int, dp, dp]),
+        "cfo_mg_vcycle": (C.c_int, [P, dp, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -297,6 +305,33 @@ class Oracle:
 
     def init_random_faces(self, seed):
         self.L.cfo_init_random_faces(self.h, seed)
+
+    # ---- multigrid (compact vectors: owned cells, x fastest) -----------------
+    def mg_nlevels(self):
+        return self.L.cfo_mg_nlevels(self.h)
+
+    def mg_level_dims(self, level):
+        n = (C.c_int * 3)()
+        rc = self.L.cfo_mg_level_dims(self.h, level, n)
+        if rc != 0:
+            raise ValueError(f"cfo_mg_level_dims rc={rc}")
+        return tuple(n)
+
+    def mg_apply(self, level, x):
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        y = np.zeros_like(x)
+        rc = self.L.cfo_mg_apply_level(self.h, level, _dp(x), _dp(y))
+        if rc != 0:
+            raise ValueError(f"cfo_mg_apply_level rc={rc}")
+        return y
+
+    def mg_vcycle(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64).ravel()
+        z = np.zeros_like(r)
+        rc = self.L.cfo_mg_vcycle(self.h, _dp(r), _dp(z))
+        if rc != 0:
+            raise ValueError(f"cfo_mg_vcycle rc={rc}")
+        return z
 
     def adjoint_run(self, nsteps, ckpt, dq_T):
         dq_T = np.ascontiguousarray(dq_T, dtype=np.float64)
