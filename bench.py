@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true",
                     help="skip the FP64 leg (profiling runs only; no JSON line)")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-mg", action="store_true",
+                    help="skip the multigrid-preconditioned FP64 leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=400)
     ap.add_argument("--timing-sample", type=int, default=8,
@@ -94,9 +96,14 @@ def timed_run(sol, steps, warmup, timing=True):
         res.append(st["final_residual"])
     sol.sync()
     t = time.perf_counter() - t0
-    kt = {k: sol.kernel_time(k) for k in range(8)}
+    kt = {k: sol.kernel_time(k) for k in range(_lib_tk_count())}
     sol.set_kernel_timing(False)
     return t, iters, outer, res, kt
+
+
+def _lib_tk_count():
+    from cajitafluids_amd import _lib
+    return _lib.TK_VCYCLE + 1
 
 
 def summarize_counters(path):
@@ -236,6 +243,9 @@ def main():
             "kernels": k32,
         }
 
+    if not args.no_mg:
+        out["multigrid"] = mg_leg(n, args, value, q64, p64)
+
     if args.adjoint:
         out["adjoint"] = adjoint_legs()
 
@@ -243,6 +253,37 @@ def main():
         out["cpu_baseline"] = cpu_baseline(cfg, n, total_iters / args.steps, args.cpu_iters)
 
     print(json.dumps(out))
+
+
+def mg_leg(n, args, value64, q64, p64):
+    """The same workload with the V-cycle preconditioner (FP64, same
+    1e-8 relative residual)."""
+    from cajitafluids_amd import _lib, configs as K
+    cfg = K.vortex_tank_3d(n)
+    cfg["preconditioner"] = K.PRECOND_MG
+    sol = make_solver(cfg)
+    t, its, _, res, kt = timed_run(sol, args.steps, args.warmup, timing=args.timing_sample)
+    qm, pm = owned_q_and_p(sol) if q64 is not None else (None, None)
+    sol.close()
+    del sol
+    v = n ** 3 * args.steps / t
+    kern = {}
+    for k, nm in ((_lib.TK_DIR_SPMV, "dir_spmv"), (_lib.TK_UPDATE, "update"),
+                  (_lib.TK_VCYCLE, "vcycle")):
+        ms, cnt = kt[k]
+        if cnt:
+            kern[nm] = {"avg_us": round(1e3 * ms / cnt, 3), "launches": cnt}
+    return {
+        "value": v, "unit": "cell-steps/s", "speedup_vs_jacobi": round(v / value64, 3),
+        "ms_per_step": 1e3 * t / args.steps,
+        "iterations_per_step": round(sum(its) / args.steps, 1),
+        "max_final_residual": max(res),
+        "mg_nu": cfg["mg_nu"], "mg_omega": cfg["mg_omega"],
+        "mg_coarse_sweeps": cfg["mg_coarse_sweeps"],
+        "dye_rel_l2_vs_jacobi": rel(qm, q64) if qm is not None else None,
+        "pressure_rel_l2_vs_jacobi": rel(pm, p64) if pm is not None else None,
+        "kernels": kern,
+    }
 
 
 def adjoint_leg(cfg, nsteps, ckpt, init, loss_grad):

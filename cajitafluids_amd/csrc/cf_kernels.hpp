@@ -15,6 +15,8 @@ struct PcgState
 {
     double rho, alpha, beta, bnorm, res, tol;
     int iter, maxit, done, converged;
+    int mg; // z comes from the multigrid V-cycle: the update kernel leaves
+            // beta and rho to k_mgpcg_rz
 };
 
 // Grid-reduction scratch of the streaming kernels: per-block double-double

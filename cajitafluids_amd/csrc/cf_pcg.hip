@@ -50,6 +50,7 @@ __device__ __forceinline__ void init_state(PcgState* st, double rz, double nn,
     st->done = zero ? 1 : (maxit <= 0 ? 1 : 0);
     st->converged = zero ? 1 : 0;
     st->res = zero ? 0.0 : 1.0;
+    st->mg = 0;
 }
 
 __global__ __launch_bounds__(256) void k_pcg_init64(
@@ -184,8 +185,10 @@ __global__ __launch_bounds__(256) void k_pcg_update(
         } else if (it >= st->maxit) {
             st->done = 1;
         } else {
-            st->beta = sm[1] / st->rho;
-            st->rho = sm[1];
+            if (!st->mg) { // MG-PCG: beta and rho follow the V-cycle
+                st->beta = sm[1] / st->rho;
+                st->rho = sm[1];
+            }
         }
     });
 }
@@ -331,8 +334,10 @@ __global__ __launch_bounds__(256) void k_pcg_update3(
         } else if (it >= st->maxit) {
             st->done = 1;
         } else {
-            st->beta = sm[1] / st->rho;
-            st->rho = sm[1];
+            if (!st->mg) { // MG-PCG: beta and rho follow the V-cycle
+                st->beta = sm[1] / st->rho;
+                st->rho = sm[1];
+            }
         }
     });
 }
@@ -430,6 +435,7 @@ __global__ __launch_bounds__(256) void k_refine_init_carry(
         st->rho = s[2];
         st->alpha = 0.0;
         st->beta = 0.0;
+        st->mg = 0;
         st->done = bnorm == 0.0 ? 1 : (maxit <= 0 ? 1 : 0);
         st->converged = bnorm == 0.0 ? 1 : 0;
         if (bnorm == 0.0)

@@ -174,6 +174,11 @@ VelocityCorrector::VelocityCorrector(ProblemManager& pm, const cf_config& cfg,
     : _pm(pm), _cfg(cfg), _s(s), _timer(timer), _g(pm.mesh().geo())
 {
     const int64_t n = _g.total;
+    _g_unit = _g;
+    for (int k = 0; k <= 6; ++k) {
+        _g_unit.dinv[k] = 1.0;
+        _g_unit.dinv32[k] = 1.0f;
+    }
     // VelocityCorrector.hpp:83-90: lhs / rhs zeroed over the ghosted extent.
     _lhs.alloc(n, s);
     _rhs.alloc(n, s);
@@ -440,8 +445,58 @@ cf_status VelocityCorrector::solve(int precision, const double* b, double* x,
     if (precision != CF_FP64)
         throw CfError(CF_EINVAL, "precision must be CF_FP64 or CF_FP32_REFINED");
     if (_cfg.preconditioner == CF_PRECOND_MG)
-        throw CfError(CF_ENOTIMPL, "the multigrid preconditioner is not built");
+        return solveMg(b, x, st);
     return solve64(b, x, st);
+}
+
+// MG-PCG: Jacobi-PCG's kernels with z = B r from one V-cycle.  The
+// direction kernel runs on a geometry whose Jacobi table is 1, so it forms
+// p = fma(beta, p_old, z) (z alone on the first iteration); the update
+// kernel leaves beta and rho to k_mgpcg_rz, which follows the V-cycle.
+// Iterations are queued in batches like runIterations; every kernel of an
+// iteration queued after convergence returns at once.
+cf_status VelocityCorrector::solveMg(const double* b, double* x, cf_solve_stats* st)
+{
+    if (!_mg)
+        _mg.reset(new Multigrid(_g, _cfg, _s));
+    RedBuf rb{_red_partials.ptr, _red_counter.ptr};
+    launch_mgpcg_init(_g, b, x, _r.ptr, _st, _cfg.cg_tol, _cfg.cg_max_iter, rb, _s);
+    const double* z = _mg->vcycle(_r.ptr, _st);
+    launch_mgpcg_rz(_g, _r.ptr, z, _st, rb, _s);
+    PcgState hs;
+    readState(hs);
+    int cur = 0, launched = 0;
+    int batch = _predict_mg > 3 ? _predict_mg - 2 : 8;
+    while (!hs.done) {
+        for (int c = 0; c < batch; ++c) {
+            const int it = hs.iter + launched + c + 1;
+            int t0 = _timer.begin_iter(_s, it);
+            launch_pcg_dir_spmv<double>(_g_unit, z, _p[cur].ptr, _p[cur ^ 1].ptr, _st, rb, _s);
+            _timer.end(t0, _s, TK_DIR_SPMV, it);
+            int t1 = _timer.begin_iter(_s, it);
+            launch_pcg_update<double>(_g, x, _r.ptr, _p[cur ^ 1].ptr, _st, rb, _s);
+            _timer.end(t1, _s, TK_UPDATE, it);
+            // the V-cycle prepares iteration it + 1 (a no-op after the stop)
+            int t2 = _timer.begin_iter(_s, it + 1);
+            z = _mg->vcycle(_r.ptr, _st);
+            _timer.end(t2, _s, TK_VCYCLE, it + 1);
+            launch_mgpcg_rz(_g, _r.ptr, z, _st, rb, _s);
+            cur ^= 1;
+        }
+        launched += batch;
+        readState(hs);
+        _timer.resolve(hs.done ? hs.iter + 1 : INT_MAX);
+        batch = 2;
+    }
+    _predict_mg = hs.iter;
+    if (st) {
+        st->converged = hs.converged;
+        st->iterations = hs.iter;
+        st->outer_iterations = 0;
+        st->b_norm = hs.bnorm;
+        st->final_residual = hs.res;
+    }
+    return hs.converged ? CF_OK : CF_ENOCONV;
 }
 
 cf_status VelocityCorrector::correctVelocity(cf_solve_stats* st)

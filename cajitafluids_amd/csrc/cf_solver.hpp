@@ -16,6 +16,7 @@
 #include "cf_api.h"
 #include "cf_geometry.hpp"
 #include "cf_kernels.hpp"
+#include "cf_mg.hpp"
 
 namespace cfmi
 {
@@ -62,7 +63,8 @@ enum TimedKernel
     TK_FINALIZE32 = 5,
     TK_ADVECT = 6,
     TK_RESIDUAL = 7,   // FP64 defect-correction residual
-    TK_COUNT = 8
+    TK_VCYCLE = 8,     // one multigrid V-cycle
+    TK_COUNT = 9
 };
 
 class KernelTimer
@@ -160,6 +162,7 @@ class VelocityCorrector
     cf_status solveRefined(const double* b, double* x, cf_solve_stats* st);
     cf_status solveRestart(const double* b, double* x, cf_solve_stats& loc);
     cf_status solveCarry(const double* b, double* x, cf_solve_stats& loc);
+    cf_status solveMg(const double* b, double* x, cf_solve_stats* st);
     template <class T>
     int runIterations(T* x, T* r, DevArray<T> (&p)[2], int& cur, int predict,
                       int tk0);
@@ -183,6 +186,10 @@ class VelocityCorrector
     PcgState* _st_host = nullptr;  // pinned
     RefineState* _rs_host = nullptr;
     int _predict64 = 0;
+    int _predict_mg = 0;
+    std::unique_ptr<Multigrid> _mg; // built on the first MG solve
+    Geo _g_unit;                    // _g with a unit Jacobi table: the
+                                    // direction kernel then reads z as is
     std::vector<int> _predict_pass; // FP32 iterations of each refinement pass
 };
 

@@ -192,9 +192,9 @@ cf_status cf_init_random_faces(cf_solver* h, uint64_t seed);
  * and accumulate per-kernel device time; the PCG kernels are timed in one
  * iteration out of every `enable` (1: every iteration).  0: off. */
 cf_status cf_set_kernel_timing(cf_solver* h, int enable);
-/* kernel: 0 FP64 direction+SpMV, 1 FP64 update, 2 FP64 finalize kernels,
- * 3-5 the same for the FP32 inner solve, 6 advection, 7 FP64
- * defect-correction residual */
+/* kernel: 0 FP64 direction+SpMV, 1 FP64 update, 2 FP64 finalize kernels
+ * (unused), 3-5 the same for the FP32 inner solve, 6 advection, 7 FP64
+ * defect-correction residual, 8 one multigrid V-cycle (all its kernels) */
 cf_status cf_kernel_time(cf_solver* h, int kernel, double* total_ms,
                          int64_t* launches);
 cf_status cf_reset_kernel_time(cf_solver* h);

@@ -75,6 +75,27 @@ def test_invalid_configs_rejected_before_any_launch(bad):
     assert ei.value.code == _lib.CF_EINVAL
 
 
+@pytest.mark.parametrize("bad", [
+    dict(preconditioner=2), dict(mg_nu=0), dict(mg_coarse_sweeps=0),
+    dict(mg_omega=0.0), dict(mg_omega=1.5),
+])
+def test_invalid_multigrid_configs(bad):
+    cfg = K.base_config(2, 8)
+    cfg.update(preconditioner=K.PRECOND_MG)
+    cfg.update(bad)
+    with pytest.raises(_lib.CfError) as ei:
+        layout(cfg, K.CELL)
+    assert ei.value.code == _lib.CF_EINVAL
+
+
+def test_multigrid_fp32_not_implemented():
+    cfg = K.base_config(2, 8)
+    cfg.update(preconditioner=K.PRECOND_MG, precision=K.FP32_REFINED)
+    with pytest.raises(_lib.CfError) as ei:
+        layout(cfg, K.CELL)
+    assert ei.value.code == _lib.CF_ENOTIMPL
+
+
 def test_create_without_device_fails_cleanly():
     try:
         import torch

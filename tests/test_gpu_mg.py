@@ -1,0 +1,181 @@
+"""The HIP multigrid-preconditioned PCG (preconditioner = CF_PRECOND_MG)
+against the oracle's MG-PCG (oracle/cf_oracle_mg.c), through the C ABI.
+
+The V-cycle kernels evaluate every expression in the oracle's order with the
+same fma() calls and the same coarse coefficients (integer face counts times
+dt / (rho dx^2), rounded once), and the PCG dots are double-double, so the
+iterates agree to rounding and the iteration counts are equal.  Truncated
+solves (1, 2 and 5 iterations) compare x after the first V-cycles, where a
+difference in one kernel would show before convergence hides it.
+"""
+import numpy as np
+import pytest
+
+from cajitafluids_amd import configs as K
+from helpers import compare_state, gpu_to_oracle, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+SOLID, FREE = K.SOLID, K.FREE
+
+
+@pytest.fixture(scope="module")
+def cf():
+    from cajitafluids_amd import solver
+    return solver
+
+
+def _cfg(dim, n, periodic=(0, 0, 0), walls=None, **kw):
+    c = K.base_config(dim, n[0])
+    c["num_cell"] = list(n) + [1] * (3 - len(n))
+    c["hi"] = [c["num_cell"][d] / n[0] if d < dim else 1.0 for d in range(3)]
+    c.update(clamp_dt=0, dt=0.002, periodic=list(periodic), preconditioner=K.PRECOND_MG)
+    if walls is not None:
+        c["wall_type"] = walls
+    c.update(kw)
+    return c
+
+
+CASES = {
+    "2d_solid_32": _cfg(2, (32, 32)),
+    "2d_free_45x27": _cfg(2, (45, 27), walls=[[SOLID, FREE], [FREE, SOLID], [SOLID, SOLID]]),
+    "2d_periodic_x_40x33": _cfg(2, (40, 33), periodic=(1, 0, 0)),
+    "2d_periodic_xy_64": _cfg(2, (64, 64), periodic=(1, 1, 0)),
+    "3d_solid_12": _cfg(3, (12, 12, 12)),
+    "3d_ragged_70x36x40": _cfg(3, (70, 36, 40), halo=2),
+    "3d_periodic_xy_21x18x11": _cfg(3, (21, 18, 11), periodic=(1, 1, 0)),
+    "3d_periodic_all_16": _cfg(3, (16, 16, 16), periodic=(1, 1, 1)),
+    "3d_free_top_24x20x17": _cfg(3, (24, 20, 17),
+                                 walls=[[SOLID, SOLID], [SOLID, SOLID], [SOLID, FREE]]),
+    "3d_nu1_coarse4_20": _cfg(3, (20, 20, 20), mg_nu=1, mg_coarse_sweeps=4, mg_omega=0.6),
+}
+
+
+def _prepare(cf, oracle_mod, cfg, seed=3):
+    sol = cf.Solver(cfg)
+    orc = oracle_mod.Oracle(cfg)
+    sol.init_random_faces(seed)
+    gpu_to_oracle(sol, orc)
+    for o in (sol, orc):
+        o.build_rhs()
+        o.rhs_remove_mean()
+    return sol, orc
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_mg_pcg_matches_oracle(cf, oracle_mod, name):
+    cfg = CASES[name]
+    print(f"mg-pcg vs oracle: {name} {cfg['num_cell']}", flush=True)
+    sol, orc = _prepare(cf, oracle_mod, cfg)
+    st_g = sol.pressure_solve(K.FP64)
+    rc, st_o = orc.pressure_solve(K.FP64)
+    assert rc == 0 and st_g["converged"]
+    assert st_g["iterations"] == st_o["iterations"]
+    assert st_g["final_residual"] < cfg["cg_tol"]
+    x_g = sol.owned_view(sol.get_pressure(0), 0)
+    x_o = orc.owned_view(orc.lhs(), 0)
+    assert rel_l2(x_g, x_o) <= TOL
+    sol.close()
+    orc.close()
+
+
+@pytest.mark.parametrize("maxit", [1, 2, 5])
+@pytest.mark.parametrize("name", ["2d_free_45x27", "3d_ragged_70x36x40",
+                                  "3d_periodic_xy_21x18x11", "3d_nu1_coarse4_20"])
+def test_mg_pcg_truncated(cf, oracle_mod, name, maxit):
+    from cajitafluids_amd import _lib
+    cfg = K.copy_config(CASES[name])
+    cfg["cg_max_iter"] = maxit
+    print(f"mg-pcg truncated: {name} maxit={maxit}", flush=True)
+    sol, orc = _prepare(cf, oracle_mod, cfg, seed=11)
+    with pytest.raises(_lib.CfError) as ei:
+        sol.pressure_solve(K.FP64)
+    assert ei.value.code == _lib.CF_ENOCONV
+    rc, st_o = orc.pressure_solve(K.FP64)
+    assert rc == _lib.CF_ENOCONV
+    assert sol.last_stats["iterations"] == st_o["iterations"] == maxit
+    x_g = sol.owned_view(sol.get_pressure(0), 0)
+    x_o = orc.owned_view(orc.lhs(), 0)
+    assert rel_l2(x_g, x_o) <= TOL
+    assert abs(sol.last_stats["final_residual"] - st_o["final_residual"]) \
+        <= 1e-12 * st_o["final_residual"]
+    sol.close()
+    orc.close()
+
+
+def test_mg_fewer_iterations_than_jacobi(cf):
+    """On a 64^3 tank the V-cycle preconditioner cuts the PCG iterations
+    several-fold against the Jacobi preconditioner, on the same right-hand
+    side, and both reach the tolerance."""
+    base = K.vortex_tank_3d(64)
+    base.update(clamp_dt=0, dt=0.002)
+    its = {}
+    for pc in (K.PRECOND_JACOBI, K.PRECOND_MG):
+        cfg = K.copy_config(base)
+        cfg["preconditioner"] = pc
+        with cf.Solver(cfg) as sol:
+            sol.init_random_faces(5)
+            sol.build_rhs()
+            sol.rhs_remove_mean()
+            st = sol.pressure_solve(K.FP64)
+            assert st["converged"] and st["final_residual"] < cfg["cg_tol"]
+            its[pc] = st["iterations"]
+    assert its[K.PRECOND_MG] * 4 <= its[K.PRECOND_JACOBI], its
+
+
+@pytest.mark.parametrize("case", ["dye_tank_2d_64", "vortex_tank_3d_16"])
+def test_mg_full_steps(cf, oracle_mod, case):
+    if case == "dye_tank_2d_64":
+        cfg, nsteps = K.dye_tank_2d(64), 6
+    else:
+        cfg, nsteps = K.vortex_tank_3d(16), 4
+    cfg["preconditioner"] = K.PRECOND_MG
+    print(f"mg full steps: {case} x{nsteps}", flush=True)
+    sol = cf.Solver(cfg)
+    orc = oracle_mod.Oracle(cfg)
+    if cfg["dim"] == 3:
+        sol.init_vortex_tubes(1)
+        orc.init_vortex_tubes(1)
+    st_g = sol.setup()
+    rc, st_o = orc.setup()
+    assert rc == 0 and st_g["iterations"] == st_o["iterations"]
+    for n in range(nsteps):
+        st_g = sol.step()
+        rc, st_o = orc.step()
+        assert rc == 0 and st_g["iterations"] == st_o["iterations"], n
+    for e, (rl2, mx) in compare_state(sol, orc).items():
+        assert rl2 <= TOL, f"entity {e}: rel L2 {rl2:.3e} (max abs {mx:.3e})"
+    sol.close()
+    orc.close()
+
+
+def test_mg_adjoint_matches_oracle(cf, oracle_mod):
+    """The reverse mode's pressure solves go through the same preconditioner
+    on both sides."""
+    cfg = K.adjoint_tank_2d(24)
+    cfg.update(cg_tol=1e-12, inflow_velocity=[0.5, 3.0, 0.0], preconditioner=K.PRECOND_MG)
+    rng = np.random.default_rng(7)
+    init = {}
+    for e in range(3):
+        ext, lo, hi = cf.layout(cfg, e)
+        a = np.zeros((ext[2], ext[1], ext[0]))
+        sl = (slice(lo[2], hi[2]), slice(lo[1], hi[1]), slice(lo[0], hi[0]))
+        a[sl] = rng.normal(size=a[sl].shape) * (0.3 if e else 1.0)
+        init[e] = a
+    w = rng.normal(size=init[0].shape)
+    with cf.Solver(cfg) as sol:
+        for e, a in init.items():
+            sol.set(e, a)
+        g = sol.adjoint_run(4, 2, w)
+    orc = oracle_mod.Oracle(cfg)
+    for e, a in init.items():
+        orc.field(e)[...] = a
+    rc, go = orc.adjoint_run(4, 2, w)
+    assert rc == 0
+    for e, k in enumerate(("dq0", "du", "dv")):
+        _, lo, hi = cf.layout(cfg, e)
+        sl = (slice(lo[2], hi[2]), slice(lo[1], hi[1]), slice(lo[0], hi[0]))
+        assert rel_l2(g[k][sl], go[k][sl]) <= 1e-10, k
+    orc.close()
+
