@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-mg", action="store_true",
                     help="skip the multigrid-preconditioned FP64 leg")
+    ap.add_argument("--mg-only", action="store_true",
+                    help="only the multigrid leg (profiling runs; prints its object)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=400)
     ap.add_argument("--timing-sample", type=int, default=8,
@@ -140,6 +142,9 @@ def main():
     if args.copy_probe:
         copy_probe()
     n = args.n
+    if args.mg_only:
+        print(json.dumps({"multigrid": mg_leg(n, args, None, None, None)}))
+        return
     ncell = n ** 3
     if args.no_fp64:
         cfg32 = K.vortex_tank_3d(n, precision=K.FP32_REFINED)
@@ -274,7 +279,8 @@ def mg_leg(n, args, value64, q64, p64):
         if cnt:
             kern[nm] = {"avg_us": round(1e3 * ms / cnt, 3), "launches": cnt}
     return {
-        "value": v, "unit": "cell-steps/s", "speedup_vs_jacobi": round(v / value64, 3),
+        "value": v, "unit": "cell-steps/s",
+        "speedup_vs_jacobi": round(v / value64, 3) if value64 else None,
         "ms_per_step": 1e3 * t / args.steps,
         "iterations_per_step": round(sum(its) / args.steps, 1),
         "max_final_residual": max(res),

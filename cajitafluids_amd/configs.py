@@ -51,6 +51,7 @@ def base_config(dim=2, n=128):
         mg_nu=2,
         mg_omega=0.8,
         mg_coarse_sweeps=16,
+        mg_alpha=1.8,
     )
 
 

@@ -62,8 +62,9 @@ cf_status build_geometry(const cf_config& c, Geo& g, const char** why)
             return CF_ENOTIMPL;
         }
         if (c.mg_nu < 1 || c.mg_coarse_sweeps < 1 || !(c.mg_omega > 0) ||
-            !(c.mg_omega <= 1)) {
-            *why = "mg_nu and mg_coarse_sweeps must be >= 1, mg_omega in (0, 1]";
+            !(c.mg_omega <= 1) || !(c.mg_alpha > 0) || !(c.mg_alpha <= 2)) {
+            *why = "mg_nu and mg_coarse_sweeps must be >= 1, mg_omega in (0, 1], "
+                   "mg_alpha in (0, 2]";
             return CF_EINVAL;
         }
         for (int d = 0; d < c.dim; ++d)

@@ -3,22 +3,41 @@
 // Arithmetic order, shared with the oracle's restatement so that the
 // V-cycle is bit-identical: A x = diag x, then fma over the -x, +x, -y, +y,
 // -z, +z couplings (a missing neighbour contributes fma(0, 0, .)); a Jacobi
-// sweep is fma(w, b - A x, x); restriction sums the children in z, y, x
-// order starting from 0; prolongation adds the parent's value.
+// sweep is fma(w, b - A x, x); restriction sums the children's residuals in
+// z, y, x order starting from 0; prolongation adds the parent's value.
+//
+// Sweeps are fused with what precedes them: the first sweep from zero reads
+// x = w b at each stencil point from b, and the first sweep after the coarse
+// correction reads fma(alpha, xc[parent], x) at each point, so neither x = w b
+// nor the prolongated x is written out.  The residual is formed inside the
+// restriction, and the last level-0 sweep also reduces r.z for the PCG.
 #include <hip/hip_runtime.h>
 
 #include "cf_mg.hpp"
 #include "cf_stream.hpp"
+#include "cf_tile3.hpp"
 
 namespace cfmi
 {
 
 // ---- level 0: padded layout, matrix-free operator ---------------------------------
 
-__device__ __forceinline__ double tab_pick(const MgTab& t, int ub, int wx)
+// table entry by total wall count without dynamic indexing of the argument
+__device__ __forceinline__ double tab_sel(const MgTab& t, int cnt)
 {
-    const double a = t.v[ub], b = t.v[ub + 1], c = t.v[ub + 2];
-    return wx == 0 ? a : (wx == 1 ? b : c);
+    double v = t.v[0];
+#pragma unroll
+    for (int q = 1; q <= 6; ++q)
+        v = cnt == q ? t.v[q] : v;
+    return v;
+}
+
+__device__ __forceinline__ int wrap_nbr(int periodic, int n, int i, int dir)
+{
+    const int m = i + dir;
+    if (periodic)
+        return m < 0 ? n - 1 : (m >= n ? 0 : m);
+    return m;
 }
 
 template <int DIM>
@@ -31,45 +50,250 @@ __global__ __launch_bounds__(256) void k_mg0_first(Geo g, MgTab wd, const double
         const int ub = wall_count(g, 1, j) + wall_count(g, 2, k);
         for (int i = threadIdx.x; i < g.n[0]; i += kBlock) {
             const int64_t c = base + i;
-            x[c] = tab_pick(wd, ub, wall_count(g, 0, i)) * b[c];
+            x[c] = tab_sel(wd, ub + wall_count(g, 0, i)) * b[c];
         }
     });
 }
 
-template <int DIM>
-__global__ __launch_bounds__(256) void k_mg0_sweep(Geo g, MgTab wd, const double* __restrict__ xi,
-                                                   const double* __restrict__ b,
-                                                   double* __restrict__ xo, const PcgState* st)
+// One Jacobi sweep xo = fma(w, b - A x, x), x from SRC; RZ adds r.z.
+template <int DIM, int SRC, bool RZ>
+__global__ __launch_bounds__(256) void k_mg0_smooth(Geo g, MgTab wd, const double* __restrict__ xi,
+                                                    const double* __restrict__ b, MgLev c1,
+                                                    const double* __restrict__ xc,
+                                                    double* __restrict__ xo, PcgState* st,
+                                                    RedBuf rb)
 {
     if (st->done)
         return;
+    double hi[1] = {0.0}, lo[1] = {0.0};
+    const double s = offdiag<double>(g);
+    // level 0 -> 1 coarsens every axis with more than 2 cells
+    const bool cx = g.n[0] > 2, cy = g.n[1] > 2, cz = DIM > 2 && g.n[2] > 2;
     for_owned_rows(g, [&](int64_t base, int j, int k) {
         const RowStencil rs = row_stencil(g, j, k);
+        // level-0 coordinates of the y / z neighbour rows
+        const int jn[2] = {wrap_nbr(g.periodic[1], g.n[1], j, -1),
+                           wrap_nbr(g.periodic[1], g.n[1], j, +1)};
+        const int kn[2] = {DIM > 2 ? wrap_nbr(g.periodic[2], g.n[2], k, -1) : 0,
+                           DIM > 2 ? wrap_nbr(g.periodic[2], g.n[2], k, +1) : 0};
+        auto parent = [&](int i0, int j0, int k0) -> int {
+            const int P0 = cx ? i0 >> 1 : i0, P1 = cy ? j0 >> 1 : j0, P2 = cz ? k0 >> 1 : k0;
+            return P0 + c1.n[0] * (P1 + c1.n[1] * P2);
+        };
+        // value at a stencil point (0 at a ghost across a wall)
+        auto val = [&](int64_t at, bool wall, int cnt, int i0, int j0, int k0) -> double {
+            if constexpr (SRC == MG_PLAIN) {
+                return xi[at];
+            } else if constexpr (SRC == MG_FIRST) {
+                return wall ? 0.0 : tab_sel(wd, cnt) * b[at];
+            } else {
+                return wall ? 0.0 : fma(c1.alpha, xc[parent(i0, j0, k0)], xi[at]);
+            }
+        };
         for (int i = threadIdx.x; i < g.n[0]; i += kBlock) {
             const int64_t c = base + i;
             const LaneX lx = lane_x(g, i);
-            const double y = apply_A<DIM, double>(g, rs, lx, xi, c);
-            xo[c] = fma(tab_pick(wd, rs.cnt, lx.cnt), b[c] - y, xi[c]);
+            const int cnt = rs.cnt + lx.cnt;
+            const double vc = val(c, false, cnt, i, j, k);
+            double y = diag_pick<double>(g, rs.cnt, lx.cnt) * vc;
+            y = fma(lx.wall[0] ? 0.0 : s,
+                    val(c + lx.off[0], lx.wall[0], rs.cnt + lx.ncnt[0], i + (int)lx.off[0], j, k), y);
+            y = fma(lx.wall[1] ? 0.0 : s,
+                    val(c + lx.off[1], lx.wall[1], rs.cnt + lx.ncnt[1], i + (int)lx.off[1], j, k), y);
+            y = fma(rs.wall[1][0] ? 0.0 : s,
+                    val(c + rs.off[1][0], rs.wall[1][0], rs.ncnt[1][0] + lx.cnt, i, jn[0], k), y);
+            y = fma(rs.wall[1][1] ? 0.0 : s,
+                    val(c + rs.off[1][1], rs.wall[1][1], rs.ncnt[1][1] + lx.cnt, i, jn[1], k), y);
+            if constexpr (DIM > 2) {
+                y = fma(rs.wall[2][0] ? 0.0 : s,
+                        val(c + rs.off[2][0], rs.wall[2][0], rs.ncnt[2][0] + lx.cnt, i, j, kn[0]),
+                        y);
+                y = fma(rs.wall[2][1] ? 0.0 : s,
+                        val(c + rs.off[2][1], rs.wall[2][1], rs.ncnt[2][1] + lx.cnt, i, j, kn[1]),
+                        y);
+            }
+            const double bc = b[c];
+            const double out = fma(tab_sel(wd, cnt), bc - y, vc);
+            xo[c] = out;
+            if constexpr (RZ)
+                dd_add_prod(hi[0], lo[0], bc, out);
         }
     });
+    if constexpr (RZ) {
+        grid_reduce<1>(hi, lo, rb, [&](double* sm) {
+            if (st->iter > 0)
+                st->beta = sm[0] / st->rho;
+            st->rho = sm[0];
+        });
+    }
 }
 
-template <int DIM>
-__global__ __launch_bounds__(256) void k_mg0_residual(Geo g, const double* __restrict__ x,
-                                                      const double* __restrict__ b,
-                                                      double* __restrict__ r,
-                                                      const PcgState* st)
+// ---- tiled levels, 3-D: z-marching tiles (cf_tile3.hpp) ----------------------------
+// Level 0 and the coarse levels whose cells are all full 2^l boxes (a padded
+// layout, like level 0, with that level's Geo: couplings -(4^l s), diagonal
+// 4^l (6 - SOLID walls) s) when every axis is coarsened further.  The tile
+// grid has an even z chunk so that the two z children of a parent are
+// marched by one block; the tile's x origin and rows are even, so the x
+// children of a parent are an even lane and the next one, and its y
+// children two rows of one lane.
+
+// Sweep epilogue: xo = fma(w, b - y, x) at each own cell, w = omega / diag
+// from gw's Jacobi table; RZ also accumulates r.z = b . xo.  RAW: b is the
+// marched operand (the first sweep), taken from the tile's loads.
+template <bool RZ, bool RAW>
+struct MgSweepEpi
 {
+    static constexpr bool kRaw = RAW;
+    const Geo& gw;
+    const double* __restrict__ b;
+    double* __restrict__ xo;
+    double hi[1] = {0.0}, lo[1] = {0.0};
+    double bc[tile3::RY], bn[tile3::RY];
+    template <class PL>
+    __device__ void take_raw(const PL& P)
+    {
+#pragma unroll
+        for (int q = 0; q < tile3::RY; ++q)
+            bn[q] = P.a[q][0];
+    }
+    __device__ void prefetch(const Tile3Ctx<1>& c, const Plane3Meta& m, int k)
+    {
+        if constexpr (!RAW) {
+#pragma unroll
+            for (int q = 0; q < tile3::RY; ++q) {
+                const bool ok = k < c.k1 && c.valid[q][0];
+                bn[q] = ok ? b[c.colb[q] + m.off] : 0.0;
+            }
+        }
+    }
+    __device__ void rotate()
+    {
+#pragma unroll
+        for (int q = 0; q < tile3::RY; ++q)
+            bc[q] = bn[q];
+    }
+    __device__ void cells(const Tile3Ctx<1>& c, const Plane3Meta& m, int q, const double (&pc)[1],
+                          const double (&y)[1])
+    {
+        const double w = dinv_pick<double>(gw, c.wy[q] + m.wz, c.wx[0]);
+        const double out = fma(w, bc[q] - y[0], pc[0]);
+        xo[c.colb[q] + m.off] = out;
+        if constexpr (RZ)
+            dd_acc_prod<false>(hi[0], lo[0], bc[q], out);
+    }
+};
+
+// MODE: 1 = first sweep from zero (values w b, read from b through gw's
+// Jacobi table), 0 = plain, 2 = after the coarse correction.
+template <int MODE, bool RZ>
+__global__ __launch_bounds__(256) void k_mg_smooth3(Geo g, Geo gw, Tile3Grid tg,
+                                                    const double* __restrict__ xi,
+                                                    const double* __restrict__ b, Tile3Parent par,
+                                                    double* __restrict__ xo, PcgState* st,
+                                                    RedBuf rb)
+{
+    using namespace tile3;
     if (st->done)
         return;
-    for_owned_rows(g, [&](int64_t base, int j, int k) {
-        const RowStencil rs = row_stencil(g, j, k);
-        for (int i = threadIdx.x; i < g.n[0]; i += kBlock) {
-            const int64_t c = base + i;
-            const LaneX lx = lane_x(g, i);
-            r[c] = b[c] - apply_A<DIM, double>(g, rs, lx, x, c);
+    __shared__ double pl[2][TY + 2][tx<1>() + 2];
+    MgSweepEpi<RZ, MODE == 1> epi{gw, b, xo};
+    if constexpr (MODE == 1)
+        tile3_march<double, 1, 1>(gw, tg, b, nullptr, true, 0.0, pl, epi);
+    else
+        tile3_march<double, MODE, 1>(g, tg, xi, nullptr, false, par.alpha, pl, epi, par);
+    if constexpr (RZ) {
+        dd_norm(epi.hi[0], epi.lo[0]);
+        grid_reduce<1>(epi.hi, epi.lo, rb, [&](double* sm) {
+            if (st->iter > 0)
+                st->beta = sm[0] / st->rho;
+            st->rho = sm[0];
+        });
+    }
+}
+
+// Residual-restriction epilogue: r = b - A x per own cell; after each plane
+// an even lane adds its own and the next lane's residuals of each row pair
+// (children in x, y order), and after the second plane of a z pair (or the
+// last plane) writes the parent.  rotate() runs on every lane of an active
+// block, so the lane exchange is not under divergent control.
+struct MgRestrictEpi
+{
+    const Geo& g;
+    const double* __restrict__ b;
+    double* __restrict__ bc;
+    MgParentMap pm;
+    const Tile3Ctx<1>* ctx = nullptr;
+    double bcur[tile3::RY], bn[tile3::RY], r[tile3::RY] = {0.0, 0.0, 0.0, 0.0};
+    double sum[tile3::RY / 2] = {0.0, 0.0};
+    int knext = -1, kcur = -1;
+    __device__ void prefetch(const Tile3Ctx<1>& c, const Plane3Meta& m, int k)
+    {
+        ctx = &c;
+        knext = k;
+#pragma unroll
+        for (int q = 0; q < tile3::RY; ++q) {
+            const bool ok = k < c.k1 && c.valid[q][0];
+            bn[q] = ok ? b[c.colb[q] + m.off] : 0.0;
         }
-    });
+    }
+    __device__ void cells(const Tile3Ctx<1>&, const Plane3Meta&, int q, const double (&)[1],
+                          const double (&y)[1])
+    {
+        r[q] = bcur[q] - y[0];
+    }
+    __device__ void rotate()
+    {
+        const Tile3Ctx<1>& c = *ctx;
+        if (kcur >= 0) {
+            const int gi = c.i0 + c.lane;
+            const bool even = (c.lane & 1) == 0;
+            const bool x1 = gi + 1 < g.n[0];
+#pragma unroll
+            for (int p = 0; p < tile3::RY / 2; ++p) {
+                const double a0 = r[2 * p], a1 = r[2 * p + 1];
+                const double b0 = __shfl_down(a0, 1), b1 = __shfl_down(a1, 1);
+                if (even && c.valid[2 * p][0]) {
+                    sum[p] = sum[p] + a0;
+                    if (x1)
+                        sum[p] = sum[p] + b0;
+                    if (c.valid[2 * p + 1][0]) {
+                        sum[p] = sum[p] + a1;
+                        if (x1)
+                            sum[p] = sum[p] + b1;
+                    }
+                }
+            }
+            if ((kcur & 1) || kcur == g.n[2] - 1) {
+#pragma unroll
+                for (int p = 0; p < tile3::RY / 2; ++p) {
+                    if (even && c.valid[2 * p][0]) {
+                        const int gj = c.j0 + c.w * tile3::RY + 2 * p;
+                        bc[pm.base + (gi >> 1) + pm.sy * (gj >> 1) + pm.sz * (kcur >> 1)] =
+                            sum[p];
+                    }
+                    sum[p] = 0.0;
+                }
+            }
+        }
+        kcur = knext;
+#pragma unroll
+        for (int q = 0; q < tile3::RY; ++q)
+            bcur[q] = bn[q];
+    }
+};
+
+__global__ __launch_bounds__(256) void k_mg_resrestrict3(Geo g, Tile3Grid tg, MgParentMap pm,
+                                                         const double* __restrict__ x,
+                                                         const double* __restrict__ b,
+                                                         double* __restrict__ bc,
+                                                         const PcgState* st)
+{
+    using namespace tile3;
+    if (st->done)
+        return;
+    __shared__ double pl[2][TY + 2][tx<1>() + 2];
+    MgRestrictEpi epi{g, b, bc, pm};
+    tile3_march<double, 0, 1>(g, tg, x, nullptr, false, 0.0, pl, epi);
 }
 
 // children of parent index P along an axis of n cells
@@ -83,26 +307,28 @@ __device__ __forceinline__ void children(bool coarsen, int P, int n, int& lo, in
     }
 }
 
-__device__ __forceinline__ void unflatten(int64_t id, const int (&n)[3], int (&c)[3])
+__device__ __forceinline__ void unflatten(int id, const int (&n)[3], int (&c)[3])
 {
-    c[0] = (int)(id % n[0]);
-    const int64_t r = id / n[0];
-    c[1] = (int)(r % n[1]);
-    c[2] = (int)(r / n[1]);
+    const unsigned u = (unsigned)id;
+    c[0] = (int)(u % (unsigned)n[0]);
+    const unsigned r = u / (unsigned)n[0];
+    c[1] = (int)(r % (unsigned)n[1]);
+    c[2] = (int)(r / (unsigned)n[1]);
 }
 
-__device__ __forceinline__ int64_t flat(const int (&n)[3], int i, int j, int k)
+__device__ __forceinline__ int flat(const int (&n)[3], int i, int j, int k)
 {
-    return (int64_t)i + (int64_t)n[0] * ((int64_t)j + (int64_t)n[1] * k);
+    return i + n[0] * (j + n[1] * k);
 }
 
-#define CF_GRID_STRIDE(id, count)                                                   \
-    for (int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; id < (count); \
-         id += (int64_t)gridDim.x * blockDim.x)
+#define CF_GRID_STRIDE(id, count) \
+    for (int id = blockIdx.x * blockDim.x + threadIdx.x; id < (count); id += gridDim.x * blockDim.x)
 
-__global__ __launch_bounds__(256) void k_mg0_restrict(Geo g, MgLev C, const double* __restrict__ r,
-                                                      double* __restrict__ bc,
-                                                      const PcgState* st)
+template <int DIM>
+__global__ __launch_bounds__(256) void k_mg0_resrestrict(Geo g, MgLev C, const double* __restrict__ x,
+                                                         const double* __restrict__ b,
+                                                         double* __restrict__ bc,
+                                                         const PcgState* st)
 {
     if (st->done)
         return;
@@ -115,56 +341,161 @@ __global__ __launch_bounds__(256) void k_mg0_restrict(Geo g, MgLev C, const doub
             children(d < g.dim && g.n[d] > 2, P[d], d < g.dim ? g.n[d] : 1, lo[d], hi[d]);
         double sum = 0.0;
         for (int k = lo[2]; k <= hi[2]; ++k)
-            for (int j = lo[1]; j <= hi[1]; ++j)
-                for (int i = lo[0]; i <= hi[0]; ++i)
-                    sum = sum + r[g.idx(g.h + i, g.h + j, g.dim == 3 ? g.h + k : 0)];
+            for (int j = lo[1]; j <= hi[1]; ++j) {
+                const RowStencil rs = row_stencil(g, j, k);
+                for (int i = lo[0]; i <= hi[0]; ++i) {
+                    const int64_t c = g.idx(g.h + i, g.h + j, DIM == 3 ? g.h + k : 0);
+                    const LaneX lx = lane_x(g, i);
+                    sum = sum + (b[c] - apply_A<DIM, double>(g, rs, lx, x, c));
+                }
+            }
         bc[id] = sum;
     }
 }
 
-__global__ __launch_bounds__(256) void k_mg0_prolong(Geo g, MgLev C, const double* __restrict__ xc,
-                                                     double* __restrict__ x, const PcgState* st)
+// ---- compact levels: matrix-free Galerkin operators -------------------------------
+
+// level-0 cells of the box of coarse index P along axis d
+__device__ __forceinline__ int agg_ext(const MgLev& L, int d, int P)
 {
-    if (st->done)
-        return;
-    for_owned_cells(g, [&](int64_t c, int i, int j, int k) {
-        const int f[3] = {i, j, k};
-        int P[3];
-        for (int d = 0; d < 3; ++d)
-            P[d] = (d < g.dim && g.n[d] > 2) ? f[d] / 2 : f[d];
-        x[c] = x[c] + xc[flat(C.n, P[0], P[1], P[2])];
-    });
+    const int lo = P << L.shift[d];
+    const int hi = min(lo + (1 << L.shift[d]), L.n0[d]);
+    return hi - lo;
 }
 
-// ---- compact levels ------------------------------------------------------------------
-
-// neighbour of cell c along d (dir -1 / +1) with the periodic wrap; -1 outside
-__device__ __forceinline__ int64_t nbr(const MgLev& L, const int (&c)[3], int d, int dir)
+// Face counts of cell c: area[d] (faces to a neighbouring box along d) and K
+// (boundary faces that are not SOLID walls).
+__device__ __forceinline__ void lev_counts(const MgLev& L, const int (&c)[3], int64_t (&area)[3],
+                                           int64_t& K)
 {
-    int m[3] = {c[0], c[1], c[2]};
+    int64_t e[3] = {1, 1, 1};
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        if (d < L.dim)
+            e[d] = agg_ext(L, d, c[d]);
+    K = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        area[d] = d == 0 ? e[1] * e[2] : (d == 1 ? e[0] * e[2] : e[0] * e[1]);
+        if (d >= L.dim)
+            continue;
+        const bool lo_cut = !L.periodic[d] && c[d] == 0 && L.solid[d][0];
+        const bool hi_cut = !L.periodic[d] && c[d] == L.n[d] - 1 && L.solid[d][1];
+        K += (lo_cut ? 0 : area[d]) + (hi_cut ? 0 : area[d]);
+    }
+}
+
+__device__ __forceinline__ double lev_w(const MgLev& L, const int (&c)[3])
+{
+    int64_t area[3], K;
+    lev_counts(L, c, area, K);
+    return L.omega / ((double)K * L.s);
+}
+
+// neighbour coordinates along d (dir -1 / +1) with the periodic wrap;
+// false outside a non-periodic axis
+__device__ __forceinline__ bool lev_nbr(const MgLev& L, const int (&c)[3], int d, int dir,
+                                        int (&m)[3])
+{
+    m[0] = c[0];
+    m[1] = c[1];
+    m[2] = c[2];
     m[d] += dir;
     if (m[d] < 0 || m[d] >= L.n[d]) {
         if (!L.periodic[d])
-            return -1;
+            return false;
         m[d] = m[d] < 0 ? L.n[d] - 1 : 0;
     }
-    return flat(L.n, m[0], m[1], m[2]);
+    return true;
 }
 
-__device__ __forceinline__ double apply_lev(const MgLev& L, const double* __restrict__ x,
-                                            int64_t id, const int (&c)[3])
+// y = A x at cell c of a compact level, x(m) from val(coords, index); also
+// returns the cell's Jacobi weight.
+template <class V>
+__device__ __forceinline__ double lev_apply(const MgLev& L, const int (&c)[3], double xc, V&& val,
+                                            double& w)
 {
-    double acc = L.diag[id] * x[id];
-    for (int d = 0; d < L.dim; ++d) {
-        const int64_t m = nbr(L, c, d, -1), p = nbr(L, c, d, +1);
-        const double cm = m >= 0 ? L.cp[d][m] : 0.0;
-        const double xm = m >= 0 ? x[m] : 0.0;
-        acc = fma(cm, xm, acc);
-        const double cpl = p >= 0 ? L.cp[d][id] : 0.0;
-        const double xp = p >= 0 ? x[p] : 0.0;
-        acc = fma(cpl, xp, acc);
+    int64_t area[3], K;
+    lev_counts(L, c, area, K);
+    const double diag = (double)K * L.s;
+    w = L.omega / diag;
+    double acc = diag * xc;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (d >= L.dim)
+            break;
+        const double ca = -((double)area[d] * L.s);
+        int m[3];
+        const bool hm = lev_nbr(L, c, d, -1, m);
+        const double xm = hm ? val(m, flat(L.n, m[0], m[1], m[2])) : 0.0;
+        acc = fma(hm ? ca : 0.0, xm, acc);
+        const bool hp = lev_nbr(L, c, d, +1, m);
+        const double xp = hp ? val(m, flat(L.n, m[0], m[1], m[2])) : 0.0;
+        acc = fma(hp ? ca : 0.0, xp, acc);
     }
     return acc;
+}
+
+__device__ __forceinline__ int lev_parent(const MgLev& F, const MgLev& C, const int (&f)[3])
+{
+    const int P0 = F.coarsen[0] ? f[0] >> 1 : f[0];
+    const int P1 = F.coarsen[1] ? f[1] >> 1 : f[1];
+    const int P2 = F.coarsen[2] ? f[2] >> 1 : f[2];
+    return flat(C.n, P0, P1, P2);
+}
+
+// The per-cell bodies, shared by the grid kernels and k_mg_tail.
+__device__ __forceinline__ void mg_first_cell(const MgLev& L, int id, const double* __restrict__ b,
+                                              double* __restrict__ x)
+{
+    int c[3];
+    unflatten(id, L.n, c);
+    x[id] = lev_w(L, c) * b[id];
+}
+
+template <int SRC>
+__device__ __forceinline__ void mg_smooth_cell(const MgLev& L, int id, const double* __restrict__ xi,
+                                               const double* __restrict__ b, const MgLev& C,
+                                               const double* __restrict__ xc,
+                                               double* __restrict__ xo)
+{
+    int c[3];
+    unflatten(id, L.n, c);
+    auto val = [&](const int (&m)[3], int mid) -> double {
+        if constexpr (SRC == MG_PLAIN)
+            return xi[mid];
+        else if constexpr (SRC == MG_FIRST)
+            return lev_w(L, m) * b[mid];
+        else
+            return fma(L.alpha, xc[lev_parent(L, C, m)], xi[mid]);
+    };
+    const double vc = val(c, id);
+    double w;
+    const double y = lev_apply(L, c, vc, val, w);
+    xo[id] = fma(w, b[id] - y, vc);
+}
+
+__device__ __forceinline__ void mg_resrestrict_cell(const MgLev& F, const MgLev& C, int id,
+                                                    const double* __restrict__ x,
+                                                    const double* __restrict__ b,
+                                                    double* __restrict__ bc)
+{
+    int P[3];
+    unflatten(id, C.n, P);
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d)
+        children(F.coarsen[d] != 0, P[d], F.n[d], lo[d], hi[d]);
+    auto val = [&](const int (&)[3], int mid) -> double { return x[mid]; };
+    double sum = 0.0;
+    for (int k = lo[2]; k <= hi[2]; ++k)
+        for (int j = lo[1]; j <= hi[1]; ++j)
+            for (int i = lo[0]; i <= hi[0]; ++i) {
+                const int c[3] = {i, j, k};
+                const int fi = flat(F.n, i, j, k);
+                double w;
+                sum = sum + (b[fi] - lev_apply(F, c, x[fi], val, w));
+            }
+    bc[id] = sum;
 }
 
 __global__ __launch_bounds__(256) void k_mg_first(MgLev L, const double* __restrict__ b,
@@ -172,64 +503,111 @@ __global__ __launch_bounds__(256) void k_mg_first(MgLev L, const double* __restr
 {
     if (st->done)
         return;
-    CF_GRID_STRIDE(id, L.ncell) { x[id] = L.w[id] * b[id]; }
+    CF_GRID_STRIDE(id, L.ncell) { mg_first_cell(L, id, b, x); }
 }
 
-__global__ __launch_bounds__(256) void k_mg_sweep(MgLev L, const double* __restrict__ xi,
-                                                  const double* __restrict__ b,
-                                                  double* __restrict__ xo, const PcgState* st)
+template <int SRC>
+__global__ __launch_bounds__(256) void k_mg_smooth(MgLev L, const double* __restrict__ xi,
+                                                   const double* __restrict__ b, MgLev C,
+                                                   const double* __restrict__ xc,
+                                                   double* __restrict__ xo, const PcgState* st)
 {
     if (st->done)
         return;
-    CF_GRID_STRIDE(id, L.ncell)
+    CF_GRID_STRIDE(id, L.ncell) { mg_smooth_cell<SRC>(L, id, xi, b, C, xc, xo); }
+}
+
+__global__ __launch_bounds__(256) void k_mg_resrestrict(MgLev F, MgLev C,
+                                                        const double* __restrict__ x,
+                                                        const double* __restrict__ b,
+                                                        double* __restrict__ bc,
+                                                        const PcgState* st)
+{
+    if (st->done)
+        return;
+    CF_GRID_STRIDE(id, C.ncell) { mg_resrestrict_cell(F, C, id, x, b, bc); }
+}
+
+// ---- the small levels in one workgroup ------------------------------------------------
+// Same cycle as the host runs on the large levels (Multigrid::vcycle), one
+// phase per sweep, __syncthreads between phases.  Every vector of these
+// levels lives in LDS (level l: X, T, B of ncell each from element
+// T->off[l]); the kernel reads level l0's right-hand side from global memory
+// and writes its cycle result to T->res[l0].
+#define CF_BLOCK_STRIDE(id, count) for (int id = threadIdx.x; id < (count); id += blockDim.x)
+
+__global__ __launch_bounds__(kMgTailThreads) void k_mg_tail(const MgTail* __restrict__ T,
+                                                            const PcgState* st)
+{
+    extern __shared__ double lds[];
+    if (st->done)
+        return;
+    const int l0 = T->l0, last = T->last, nu = T->nu, coarse = T->coarse;
+    auto X = [&](int l) { return lds + T->off[l]; };
+    auto Tv = [&](int l) { return lds + T->off[l] + T->lev[l].ncell; };
+    auto Bv = [&](int l) { return lds + T->off[l] + 2 * T->lev[l].ncell; };
+    auto res = [&](int l) { return (mg_level_swaps(l == last, nu, coarse) & 1) ? Tv(l) : X(l); };
     {
-        int c[3];
-        unflatten(id, L.n, c);
-        const double y = apply_lev(L, xi, id, c);
-        xo[id] = fma(L.w[id], b[id] - y, xi[id]);
+        const double* bg = T->B[l0];
+        double* b = Bv(l0);
+        CF_BLOCK_STRIDE(id, T->lev[l0].ncell) { b[id] = bg[id]; }
+        __syncthreads();
     }
-}
-
-__global__ __launch_bounds__(256) void k_mg_residual_restrict(MgLev F, MgLev C,
-                                                              const double* __restrict__ x,
-                                                              const double* __restrict__ b,
-                                                              double* __restrict__ bc,
-                                                              const PcgState* st)
-{
-    if (st->done)
-        return;
-    CF_GRID_STRIDE(id, C.ncell)
-    {
-        int P[3];
-        unflatten(id, C.n, P);
-        int lo[3], hi[3];
-        for (int d = 0; d < 3; ++d)
-            children(F.coarsen[d] != 0, P[d], F.n[d], lo[d], hi[d]);
-        double sum = 0.0;
-        for (int k = lo[2]; k <= hi[2]; ++k)
-            for (int j = lo[1]; j <= hi[1]; ++j)
-                for (int i = lo[0]; i <= hi[0]; ++i) {
-                    const int c[3] = {i, j, k};
-                    const int64_t fi = flat(F.n, i, j, k);
-                    sum = sum + (b[fi] - apply_lev(F, x, fi, c));
-                }
-        bc[id] = sum;
+    // down: pre-smoothing and restriction
+    for (int l = l0; l <= last; ++l) {
+        const MgLev L = T->lev[l];
+        const MgLev& C = L; // unused parent level of the non-prolong sweeps
+        const double* b = Bv(l);
+        double* x = X(l);
+        double* t = Tv(l);
+        const int pre = l == last ? coarse : nu;
+        if (pre == 1) {
+            CF_BLOCK_STRIDE(id, L.ncell) { mg_first_cell(L, id, b, x); }
+        } else {
+            CF_BLOCK_STRIDE(id, L.ncell) { mg_smooth_cell<MG_FIRST>(L, id, nullptr, b, C, nullptr, x); }
+        }
+        __syncthreads();
+        for (int it = 2; it < pre; ++it) {
+            CF_BLOCK_STRIDE(id, L.ncell) { mg_smooth_cell<MG_PLAIN>(L, id, x, b, C, nullptr, t); }
+            __syncthreads();
+            double* tmp = x;
+            x = t;
+            t = tmp;
+        }
+        if (l < last) {
+            const MgLev Cn = T->lev[l + 1];
+            double* bn = Bv(l + 1);
+            CF_BLOCK_STRIDE(id, Cn.ncell) { mg_resrestrict_cell(L, Cn, id, x, b, bn); }
+            __syncthreads();
+        }
     }
-}
-
-__global__ __launch_bounds__(256) void k_mg_prolong(MgLev F, MgLev C, const double* __restrict__ xc,
-                                                    double* __restrict__ x, const PcgState* st)
-{
-    if (st->done)
-        return;
-    CF_GRID_STRIDE(id, F.ncell)
+    // up: prolongation fused into the first post-sweep
+    for (int l = last - 1; l >= l0; --l) {
+        const MgLev L = T->lev[l];
+        const MgLev Cn = T->lev[l + 1];
+        const double* b = Bv(l);
+        // the pre-smoothing result: X after an even number of swaps
+        const bool odd = (nu >= 2 ? nu - 2 : 0) & 1;
+        double* x = odd ? Tv(l) : X(l);
+        double* t = odd ? X(l) : Tv(l);
+        const double* xc = res(l + 1);
+        CF_BLOCK_STRIDE(id, L.ncell) { mg_smooth_cell<MG_PROLONG>(L, id, x, b, Cn, xc, t); }
+        __syncthreads();
+        double* tmp = x;
+        x = t;
+        t = tmp;
+        for (int it = 1; it < nu; ++it) {
+            CF_BLOCK_STRIDE(id, L.ncell) { mg_smooth_cell<MG_PLAIN>(L, id, x, b, L, nullptr, t); }
+            __syncthreads();
+            tmp = x;
+            x = t;
+            t = tmp;
+        }
+    }
     {
-        int f[3];
-        unflatten(id, F.n, f);
-        int P[3];
-        for (int d = 0; d < 3; ++d)
-            P[d] = F.coarsen[d] ? f[d] / 2 : f[d];
-        x[id] = x[id] + xc[flat(C.n, P[0], P[1], P[2])];
+        const double* x = res(l0);
+        double* xg = T->res[l0];
+        CF_BLOCK_STRIDE(id, T->lev[l0].ncell) { xg[id] = x[id]; }
     }
 }
 
@@ -303,30 +681,107 @@ void launch_mg0_first(const Geo& g, const MgTab& wd0, const double* b, double* x
     CF_MG0_LAUNCH(k_mg0_first, g, wd0, b, x, st);
 }
 
-void launch_mg0_sweep(const Geo& g, const MgTab& wd0, const double* xi, const double* b,
-                      double* xo, const PcgState* st, hipStream_t s)
+template <int DIM>
+static void mg0_smooth_dim(const Geo& g, const MgTab& wd0, int src, bool rz, const double* xi,
+                           const double* b, const MgLev& c1, const double* xc, double* xo,
+                           PcgState* st, RedBuf rb, hipStream_t s)
 {
-    CF_MG0_LAUNCH(k_mg0_sweep, g, wd0, xi, b, xo, st);
+    const dim3 grid(stream_blocks(g)), blk(kBlock);
+#define CF_MG0_SMOOTH(SRC, RZ) \
+    hipLaunchKernelGGL((k_mg0_smooth<DIM, SRC, RZ>), grid, blk, 0, s, g, wd0, xi, b, c1, xc, xo, st, rb)
+    if (src == MG_FIRST) {
+        if (rz)
+            CF_MG0_SMOOTH(MG_FIRST, true);
+        else
+            CF_MG0_SMOOTH(MG_FIRST, false);
+    } else if (src == MG_PLAIN) {
+        if (rz)
+            CF_MG0_SMOOTH(MG_PLAIN, true);
+        else
+            CF_MG0_SMOOTH(MG_PLAIN, false);
+    } else {
+        if (rz)
+            CF_MG0_SMOOTH(MG_PROLONG, true);
+        else
+            CF_MG0_SMOOTH(MG_PROLONG, false);
+    }
+#undef CF_MG0_SMOOTH
 }
 
-void launch_mg0_residual(const Geo& g, const double* x, const double* b, double* r,
-                         const PcgState* st, hipStream_t s)
+void launch_mg0_smooth(const Geo& g, const MgTab& wd0, int src, bool rz, const double* xi,
+                       const double* b, const MgLev& c1, const double* xc, double* xo,
+                       PcgState* st, RedBuf rb, hipStream_t s)
 {
-    CF_MG0_LAUNCH(k_mg0_residual, g, x, b, r, st);
+    if (g.dim == 2)
+        mg0_smooth_dim<2>(g, wd0, src, rz, xi, b, c1, xc, xo, st, rb, s);
+    else
+        mg0_smooth_dim<3>(g, wd0, src, rz, xi, b, c1, xc, xo, st, rb, s);
 }
 
-void launch_mg0_restrict(const Geo& g, const MgLev& c1, const double* r, double* b1,
-                         const PcgState* st, hipStream_t s)
+// the tile grid of the tiled multigrid kernels: an even z chunk
+// (shorter on small levels, whose few tiles would otherwise leave most CUs
+// idle while each block marches its planes one after the other)
+static Tile3Grid mg_tile_grid(const Geo& g)
 {
-    hipLaunchKernelGGL(k_mg0_restrict, compact_grid(c1.ncell), dim3(kBlock), 0, s, g, c1, r,
-                       b1, st);
+    Tile3Grid t = tile3_grid<1>(g);
+    t.kz += t.kz & 1;
+    while (t.kz > 2 && t.tiles_x * t.tiles_y * ((g.n[2] + t.kz - 1) / t.kz) < 1024)
+        t.kz = (t.kz / 2 + 1) & ~1;
+    t.nchunk = (g.n[2] + t.kz - 1) / t.kz;
+    t.work = t.tiles_x * t.tiles_y * t.nchunk;
+    return t;
 }
 
-void launch_mg0_prolong(const Geo& g, const MgLev& c1, const double* x1, double* x,
-                        const PcgState* st, hipStream_t s)
+bool mg_tiled3(const Geo& g)
 {
-    hipLaunchKernelGGL(k_mg0_prolong, dim3(stream_blocks(g)), dim3(kBlock), 0, s, g, c1, x1,
-                       x, st);
+    return g.dim == 3 && g.n[0] > 2 && g.n[1] > 2 && g.n[2] > 2;
+}
+
+void launch_mg_smooth3(const Geo& g, const Geo& gw, int src, bool rz, const double* xi,
+                       const double* b, const double* xc, MgParentMap pm, double alpha,
+                       double* xo, PcgState* st, RedBuf rb, hipStream_t s)
+{
+    const Tile3Grid tg = mg_tile_grid(g);
+    const dim3 grid((tg.work + 7) / 8 * 8), blk(kBlock);
+    const Tile3Parent par{xc, pm.base, pm.sy, pm.sz, alpha};
+#define CF_MG_SMOOTH3(M, RZ) \
+    hipLaunchKernelGGL((k_mg_smooth3<M, RZ>), grid, blk, 0, s, g, gw, tg, xi, b, par, xo, st, rb)
+    if (src == MG_FIRST) {
+        if (rz)
+            CF_MG_SMOOTH3(1, true);
+        else
+            CF_MG_SMOOTH3(1, false);
+    } else if (src == MG_PLAIN) {
+        if (rz)
+            CF_MG_SMOOTH3(0, true);
+        else
+            CF_MG_SMOOTH3(0, false);
+    } else {
+        if (rz)
+            CF_MG_SMOOTH3(2, true);
+        else
+            CF_MG_SMOOTH3(2, false);
+    }
+#undef CF_MG_SMOOTH3
+}
+
+void launch_mg_resrestrict3(const Geo& g, const double* x, const double* b, double* bc,
+                            MgParentMap pm, const PcgState* st, hipStream_t s)
+{
+    const Tile3Grid tg = mg_tile_grid(g);
+    hipLaunchKernelGGL(k_mg_resrestrict3, dim3((tg.work + 7) / 8 * 8), dim3(kBlock), 0, s, g, tg,
+                       pm, x, b, bc, st);
+}
+
+void launch_mg0_resrestrict(const Geo& g, const MgLev& c1, const double* x, const double* b,
+                            double* b1, const PcgState* st, hipStream_t s)
+{
+    if (g.dim == 2)
+        hipLaunchKernelGGL(k_mg0_resrestrict<2>, compact_grid(c1.ncell), dim3(kBlock), 0, s, g,
+                           c1, x, b, b1, st);
+    else
+        hipLaunchKernelGGL(k_mg0_resrestrict<3>, compact_grid(c1.ncell), dim3(kBlock), 0, s, g,
+                           c1, x, b, b1, st);
 }
 
 void launch_mg_first(const MgLev& L, const double* b, double* x, const PcgState* st,
@@ -335,26 +790,29 @@ void launch_mg_first(const MgLev& L, const double* b, double* x, const PcgState*
     hipLaunchKernelGGL(k_mg_first, compact_grid(L.ncell), dim3(kBlock), 0, s, L, b, x, st);
 }
 
-void launch_mg_sweep(const MgLev& L, const double* xi, const double* b, double* xo,
-                     const PcgState* st, hipStream_t s)
+void launch_mg_smooth(const MgLev& L, int src, const double* xi, const double* b,
+                      const MgLev& C, const double* xc, double* xo, const PcgState* st,
+                      hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mg_sweep, compact_grid(L.ncell), dim3(kBlock), 0, s, L, xi, b, xo,
-                       st);
+    const dim3 grid = compact_grid(L.ncell), blk(kBlock);
+    if (src == MG_FIRST)
+        hipLaunchKernelGGL(k_mg_smooth<MG_FIRST>, grid, blk, 0, s, L, xi, b, C, xc, xo, st);
+    else if (src == MG_PLAIN)
+        hipLaunchKernelGGL(k_mg_smooth<MG_PLAIN>, grid, blk, 0, s, L, xi, b, C, xc, xo, st);
+    else
+        hipLaunchKernelGGL(k_mg_smooth<MG_PROLONG>, grid, blk, 0, s, L, xi, b, C, xc, xo, st);
 }
 
-void launch_mg_residual_restrict(const MgLev& F, const MgLev& C, const double* x,
-                                 const double* b, double* bc, const PcgState* st,
-                                 hipStream_t s)
+void launch_mg_resrestrict(const MgLev& F, const MgLev& C, const double* x, const double* b,
+                           double* bc, const PcgState* st, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mg_residual_restrict, compact_grid(C.ncell), dim3(kBlock), 0, s, F,
-                       C, x, b, bc, st);
+    hipLaunchKernelGGL(k_mg_resrestrict, compact_grid(C.ncell), dim3(kBlock), 0, s, F, C, x, b,
+                       bc, st);
 }
 
-void launch_mg_prolong(const MgLev& F, const MgLev& C, const double* xc, double* x,
-                       const PcgState* st, hipStream_t s)
+void launch_mg_tail(const MgTail* t_dev, size_t lds_bytes, const PcgState* st, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mg_prolong, compact_grid(F.ncell), dim3(kBlock), 0, s, F, C, xc, x,
-                       st);
+    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kMgTailThreads), lds_bytes, s, t_dev, st);
 }
 
 void launch_mgpcg_init(const Geo& g, const double* b, double* x, double* r, PcgState* st,

@@ -4,9 +4,16 @@
 //
 // Level 0 is the pressure grid in the handle's padded layout with the
 // matrix-free operator.  Coarser levels are compact arrays (x fastest, no
-// ghosts) with stored Galerkin coefficients: diag, and the coupling cp[d]
-// of each cell to its +e_d neighbour (wrapped on a periodic axis, 0 at a
-// domain end); the -e_d coupling of a cell is cp[d] of that neighbour.
+// ghosts).  Their Galerkin operators P^T A P (P piecewise constant over the
+// aggregation) are matrix-free too: a coarse cell is a box of level-0 cells,
+// every level-0 coupling is s = dt / (rho dx^2), so
+//   coupling to the -/+ e_d neighbour = -(area_d s)   (0 without neighbour)
+//   diagonal                         = K s,
+// where area_d is the number of level-0 faces between two neighbouring boxes
+// along d (the product of the box's extents along the other axes) and K
+// counts the box's boundary faces that are not SOLID domain walls.  These
+// are the integer face counts the aggregation produces level by level,
+// rounded once; the Jacobi weight is omega / (K s).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -20,6 +27,10 @@ namespace cfmi
 {
 
 constexpr int kMgMaxLevels = 24;
+// The coarsest levels whose vectors (3 per level) fit this much LDS run
+// inside one workgroup (k_mg_tail).
+constexpr int kMgTailLdsBytes = 64 * 1024;
+constexpr int kMgTailThreads = 1024;
 
 // omega * (1 / diag) on level 0, by the cell's SOLID wall count.
 struct MgTab
@@ -33,39 +44,82 @@ struct MgLev
     int n[3];
     int coarsen[3];   // axis coarsened from this level to the next
     int periodic[3];
+    int shift[3];     // a cell spans 2^shift[d] level-0 cells along d (fewer at the end)
+    int n0[3];        // level-0 cells per axis
+    int solid[3][2];  // SOLID domain ends of non-periodic axes
     int dim;
-    int64_t ncell;
-    const double* diag;
-    const double* cp[3];
-    const double* w;  // omega / diag
+    int ncell;
+    double s;         // dt / (rho dx^2)
+    double omega;
+    double alpha;     // coarse-correction scale: x = fma(alpha, x_parent, x)
 };
 
-// Level 0 (padded layout): x = wd0[walls] * b; one damped Jacobi sweep
-// xo = fma(wd0, b - A xi, xi); the residual r = b - A x.
+// The levels that k_mg_tail runs (l0 .. last): their LDS offsets (X, T, B
+// of level l from element off[l]), and the global arrays it reads (B[l0])
+// and writes (res[l0]).  Each level starts a cycle in X and ends it in X or
+// T by the parity of its sweep count.
+struct MgTail
+{
+    int l0, last, nu, coarse;
+    MgLev lev[kMgMaxLevels];
+    int off[kMgMaxLevels];
+    double* B[kMgMaxLevels];
+    double* res[kMgMaxLevels];
+};
+
+// Source of the values a smoothing sweep reads:
+//   first:   x = w b (the sweep from zero, computed on the fly)
+//   plain:   x = xi
+//   prolong: x = fma(alpha, xc[parent], xi)
+enum MgSrc
+{
+    MG_FIRST = 0,
+    MG_PLAIN = 1,
+    MG_PROLONG = 2
+};
+
+// Level 0 (padded layout): xo = fma(w, b - A x, x) with x from `src`; with
+// rz, also r.z = b.xo reduced into st (beta = r.z / rho after the first
+// iteration, rho = r.z).  launch_mg0_first: x = w b alone (a single sweep
+// from zero).
+void launch_mg0_smooth(const Geo& g, const MgTab& wd0, int src, bool rz, const double* xi,
+                       const double* b, const MgLev& c1, const double* xc, double* xo,
+                       PcgState* st, RedBuf rb, hipStream_t s);
 void launch_mg0_first(const Geo& g, const MgTab& wd0, const double* b, double* x,
                       const PcgState* st, hipStream_t s);
-void launch_mg0_sweep(const Geo& g, const MgTab& wd0, const double* xi,
-                      const double* b, double* xo, const PcgState* st, hipStream_t s);
-void launch_mg0_residual(const Geo& g, const double* x, const double* b, double* r,
-                         const PcgState* st, hipStream_t s);
-// Level 0 -> 1: b1 = sum over children of r (z, y, x order).
-void launch_mg0_restrict(const Geo& g, const MgLev& c1, const double* r, double* b1,
-                         const PcgState* st, hipStream_t s);
-// Level 1 -> 0: x += x1[parent].
-void launch_mg0_prolong(const Geo& g, const MgLev& c1, const double* x1, double* x,
-                        const PcgState* st, hipStream_t s);
+// Level 0 -> 1: b1 = sum over children (z, y, x order) of (b - A x).
+void launch_mg0_resrestrict(const Geo& g, const MgLev& c1, const double* x, const double* b,
+                            double* b1, const PcgState* st, hipStream_t s);
+
+// Linear index of the parent of cell (i, j, k) in the next level's array:
+// base + i/2 + sy j/2 + sz k/2 (compact or padded layout).
+struct MgParentMap
+{
+    int64_t base, sy, sz;
+};
+
+// Tiled levels (3-D, every axis coarsened further): level 0 and the coarse
+// levels of full 2^l boxes in a padded layout with their own Geo.  The same
+// sweeps (the parent read through pm, scaled by alpha) and
+// residual-restriction as z-marching tiles; gw is the level's geometry with
+// the Jacobi table omega / diag.
+bool mg_tiled3(const Geo& g);
+void launch_mg_smooth3(const Geo& g, const Geo& gw, int src, bool rz, const double* xi,
+                       const double* b, const double* xc, MgParentMap pm, double alpha,
+                       double* xo, PcgState* st, RedBuf rb, hipStream_t s);
+void launch_mg_resrestrict3(const Geo& g, const double* x, const double* b, double* bc,
+                            MgParentMap pm, const PcgState* st, hipStream_t s);
 
 // Compact levels.
 void launch_mg_first(const MgLev& L, const double* b, double* x, const PcgState* st,
                      hipStream_t s);
-void launch_mg_sweep(const MgLev& L, const double* xi, const double* b, double* xo,
-                     const PcgState* st, hipStream_t s);
-// Level f -> f+1: bc = sum over children of (b - A x).
-void launch_mg_residual_restrict(const MgLev& F, const MgLev& C, const double* x,
-                                 const double* b, double* bc, const PcgState* st,
-                                 hipStream_t s);
-void launch_mg_prolong(const MgLev& F, const MgLev& C, const double* xc, double* x,
-                       const PcgState* st, hipStream_t s);
+void launch_mg_smooth(const MgLev& L, int src, const double* xi, const double* b,
+                      const MgLev& C, const double* xc, double* xo, const PcgState* st,
+                      hipStream_t s);
+void launch_mg_resrestrict(const MgLev& F, const MgLev& C, const double* x, const double* b,
+                           double* bc, const PcgState* st, hipStream_t s);
+// One workgroup runs the V-cycle of levels t->l0 .. t->last in lds_bytes of LDS.
+void launch_mg_tail(const MgTail* t_dev, size_t lds_bytes, const PcgState* st, hipStream_t s);
 
 // MG-PCG: r = b, x = 0, ||b|| (sets st->mg); and rho / beta from r.z.
 void launch_mgpcg_init(const Geo& g, const double* b, double* x, double* r,
@@ -79,20 +133,40 @@ class Multigrid
   public:
     Multigrid(const Geo& g, const cf_config& cfg, hipStream_t s);
     ~Multigrid();
-    // z = B r (one V-cycle); returns the level-0 buffer that holds z (one of
-    // the two the hierarchy owns).  Kernels return at once when st->done.
-    const double* vcycle(const double* r, const PcgState* st);
+    // z = B r (one V-cycle), then rho / beta from r.z; returns the level-0
+    // buffer that holds z (one of the two the hierarchy owns).  Kernels
+    // return at once when st->done.
+    const double* vcycle(const double* r, PcgState* st, RedBuf rb);
     int levels() const { return (int)_lev.size(); }
     const MgLev& level(int l) const { return _lev[l]; }
 
   private:
     struct Buffers;
+    void first(int l, const double* b, double* x, PcgState* st);
+    void smooth(int l, int src, bool rz, const double* xi, const double* b, double* xo,
+                PcgState* st, RedBuf rb);
+    void resrestrict(int l, const double* x, const double* b, PcgState* st);
     const Geo& _g;
     hipStream_t _s;
     int _nu, _coarse;
-    std::vector<MgLev> _lev; // _lev[0]: dims only
+    int _tail0;              // first level run by k_mg_tail
+    size_t _tail_lds;        // its LDS bytes
+    int _nt;                 // levels 0 .. _nt-1 run the z-marching tile kernels
+    std::vector<Geo> _lg;    // Geo of each tiled level (padded layout)
+    std::vector<Geo> _lgw;   // ... with the Jacobi table omega / diag
+    std::vector<MgTab> _wt;  // level 0 and tiled levels: omega / diag by wall count
+    std::vector<MgParentMap> _pm; // parent index of level l's cells in level l + 1
+    std::vector<MgLev> _lev; // _lev[0]: level-0 dims and coarsening only
     std::unique_ptr<Buffers> _buf;
-    void cycle(int l, const double* b, const PcgState* st);
 };
+
+// Sweep-count parity of one cycle on a level: the cycle starts in X and
+// ends in T when odd.
+__host__ __device__ inline int mg_level_swaps(bool last, int nu, int coarse)
+{
+    if (last)
+        return coarse >= 2 ? coarse - 2 : 0;
+    return (nu >= 2 ? nu - 2 : 0) + nu;
+}
 
 } // namespace cfmi

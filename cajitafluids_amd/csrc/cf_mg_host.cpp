@@ -3,11 +3,24 @@
 // Levels: level 0 is the pressure grid; an axis with more than 2 cells is
 // coarsened by 2 (cells 2I and 2I+1 form cell I; the last cell of an odd
 // axis stands alone), other axes are kept, until every axis has at most 2
-// cells.  Coarse operators are Galerkin P^T A P for the piecewise-constant
-// P.  Every coefficient of the pressure operator is an integer multiple of
-// s = dt / (rho dx^2), so the coarse ones are formed as integer face counts
-// (K for the diagonal, C_d for the coupling to the +e_d neighbour) and
-// rounded once: diag = K s, cp[d] = -(C_d s), w = omega / diag.
+// cells (at most kMgMaxLevels levels).  A coarse cell is then a box of
+// level-0 cells, which is all its matrix-free Galerkin operator needs.
+//
+// Three kinds of level:
+//   tiled    (3-D) level 0, and the levels below it whose boxes are all full
+//            2^l cubes while every axis still coarsens: padded layout and a
+//            Geo of their own (couplings -(4^l s), diagonal 4^l (6 - SOLID
+//            walls) s, the coarse-level values bit for bit), run by the
+//            z-marching tile kernels
+//   compact  the other large levels (and level 0 of 2-D or thin grids, in
+//            its padded layout, by the row kernels)
+//   tail     the coarsest levels, as many as fit kMgTailLdsBytes of LDS, in
+//            one workgroup
+// One V-cycle launches the tiled and compact levels from here.  Each level's
+// cycle starts in its X buffer and ends in X or T by the parity of its sweep
+// count, fixed per level, so the prolongation of the level above knows
+// where to read.
+#include <climits>
 #include <cstring>
 #include <utility>
 
@@ -19,230 +32,281 @@ namespace cfmi
 
 struct Multigrid::Buffers
 {
-    DevArray<double> z[2], r0;
-    std::vector<std::unique_ptr<DevArray<double>>> coef, vec;
-    std::vector<const double*> xcur; // per level: the buffer holding x
-    std::vector<double*> x, t, b;    // per compact level
-    MgTab wd0;
+    DevArray<double> z[2];
+    std::vector<std::unique_ptr<DevArray<double>>> vec;
+    std::vector<double*> X, T, B, res;
+    MgTail* tail = nullptr; // device copy
+    ~Buffers()
+    {
+        if (tail)
+            (void)hipFree(tail);
+    }
 };
 
 namespace
 {
 
-int64_t flat3(const int (&n)[3], int i, int j, int k)
+// Padded Geo of tiled coarse level l (3-D; every box 2^l cells a side).
+void tiled_level_geo(const Geo& g, const MgLev& L, int l, double omega, Geo& lg, Geo& lgw)
 {
-    return (int64_t)i + (int64_t)n[0] * ((int64_t)j + (int64_t)n[1] * k);
+    lg = Geo{};
+    lg.dim = 3;
+    lg.h = 1;
+    for (int d = 0; d < 3; ++d) {
+        lg.n[d] = L.n[d];
+        lg.periodic[d] = g.periodic[d];
+        lg.solid[d][0] = g.solid[d][0];
+        lg.solid[d][1] = g.solid[d][1];
+    }
+    lg.sx = (16 - (lg.h % 16)) % 16;
+    const int64_t row = lg.sx + lg.n[0] + 2 * lg.h + 1;
+    lg.px = (row + 15) / 16 * 16;
+    lg.py = lg.n[1] + 2 * lg.h + 1;
+    lg.pz = lg.n[2] + 2 * lg.h + 1;
+    lg.total = lg.px * lg.py * lg.pz + 16;
+    const int64_t A = (int64_t)1 << (2 * l); // level-0 faces per box face
+    lg.s = (double)A * g.s;
+    lg.s32 = (float)lg.s;
+    lgw = lg;
+    for (int k = 0; k <= 6; ++k) {
+        const int64_t K = A * (6 - k);
+        lg.diag[k] = (double)K * g.s;
+        lg.dinv[k] = K > 0 ? 1.0 / lg.diag[k] : 0.0;
+        lgw.diag[k] = lg.diag[k];
+        lgw.dinv[k] = K > 0 ? omega / lg.diag[k] : 0.0;
+    }
 }
 
-// index of the +e_d neighbour (wrapped when periodic), or -1
-int64_t plus_nbr(const int (&n)[3], const int (&per)[3], const int (&c)[3], int d)
+MgParentMap padded_map(const Geo& lg)
 {
-    int m[3] = {c[0], c[1], c[2]};
-    m[d] += 1;
-    if (m[d] >= n[d]) {
-        if (!per[d])
-            return -1;
-        m[d] = 0;
-    }
-    return flat3(n, m[0], m[1], m[2]);
+    return MgParentMap{lg.idx(lg.h, lg.h, lg.h), lg.px, lg.px * lg.py};
+}
+
+MgParentMap compact_map(const MgLev& L)
+{
+    return MgParentMap{0, L.n[0], (int64_t)L.n[0] * L.n[1]};
 }
 
 } // namespace
 
 Multigrid::Multigrid(const Geo& g, const cf_config& cfg, hipStream_t s)
-    : _g(g), _s(s), _nu(cfg.mg_nu), _coarse(cfg.mg_coarse_sweeps), _buf(new Buffers)
+    : _g(g), _s(s), _nu(cfg.mg_nu), _coarse(cfg.mg_coarse_sweeps), _tail_lds(0),
+      _buf(new Buffers)
 {
     const int D = g.dim;
-    int per[3] = {0, 0, 0};
-    for (int d = 0; d < D; ++d)
-        per[d] = g.periodic[d];
-    // level-0 integer counts
-    int n0[3] = {1, 1, 1};
-    for (int d = 0; d < D; ++d)
-        n0[d] = g.n[d];
-    std::vector<int64_t> K((size_t)n0[0] * n0[1] * n0[2]);
-    std::vector<int64_t> Cc[3];
-    for (int d = 0; d < D; ++d)
-        Cc[d].assign(K.size(), 0);
-    for (int k = 0; k < n0[2]; ++k)
-        for (int j = 0; j < n0[1]; ++j)
-            for (int i = 0; i < n0[0]; ++i) {
-                const int c[3] = {i, j, k};
-                const int64_t id = flat3(n0, i, j, k);
-                int64_t kd = 2 * D;
-                for (int d = 0; d < D; ++d) {
-                    if (!per[d]) {
-                        if (c[d] == 0 && g.solid[d][0])
-                            --kd;
-                        if (c[d] == n0[d] - 1 && g.solid[d][1])
-                            --kd;
-                    }
-                    Cc[d][id] = plus_nbr(n0, per, c, d) >= 0 ? 1 : 0;
-                }
-                K[id] = kd;
-            }
     MgLev L0{};
     for (int d = 0; d < 3; ++d) {
-        L0.n[d] = n0[d];
-        L0.periodic[d] = per[d];
-        L0.coarsen[d] = d < D && n0[d] > 2;
+        L0.n[d] = d < D ? g.n[d] : 1;
+        L0.n0[d] = L0.n[d];
+        L0.periodic[d] = d < D ? g.periodic[d] : 0;
+        L0.coarsen[d] = d < D && L0.n[d] > 2;
+        L0.shift[d] = 0;
+        L0.solid[d][0] = d < D && !L0.periodic[d] && g.solid[d][0];
+        L0.solid[d][1] = d < D && !L0.periodic[d] && g.solid[d][1];
     }
     L0.dim = D;
-    L0.ncell = (int64_t)K.size();
+    const int64_t n0cells = (int64_t)L0.n[0] * L0.n[1] * L0.n[2];
+    L0.ncell = n0cells <= INT_MAX ? (int)n0cells : 0; // level 0 is addressed by Geo
+    L0.s = g.s;
+    L0.omega = cfg.mg_omega;
+    L0.alpha = cfg.mg_alpha;
     _lev.push_back(L0);
-    // coarser levels
-    const double sc = g.s;
-    const double omega = cfg.mg_omega;
     while ((int)_lev.size() < kMgMaxLevels) {
-        const MgLev& F = _lev.back();
+        const MgLev F = _lev.back();
         if (!(F.coarsen[0] || F.coarsen[1] || F.coarsen[2]))
             break;
-        MgLev C{};
+        MgLev C = F;
+        int64_t nc = 1;
         for (int d = 0; d < 3; ++d) {
             C.n[d] = F.coarsen[d] ? (F.n[d] + 1) / 2 : F.n[d];
-            C.periodic[d] = per[d];
-        }
-        C.dim = D;
-        C.ncell = (int64_t)C.n[0] * C.n[1] * C.n[2];
-        for (int d = 0; d < 3; ++d)
+            C.shift[d] = F.shift[d] + (F.coarsen[d] ? 1 : 0);
             C.coarsen[d] = d < D && C.n[d] > 2;
-        std::vector<int64_t> Kc((size_t)C.ncell, 0), Ccc[3];
-        for (int d = 0; d < D; ++d)
-            Ccc[d].assign((size_t)C.ncell, 0);
-        for (int Kz = 0; Kz < C.n[2]; ++Kz)
-            for (int J = 0; J < C.n[1]; ++J)
-                for (int I = 0; I < C.n[0]; ++I) {
-                    const int P[3] = {I, J, Kz};
-                    int lo[3], hi[3];
-                    for (int d = 0; d < 3; ++d) {
-                        if (F.coarsen[d]) {
-                            lo[d] = 2 * P[d];
-                            hi[d] = 2 * P[d] + 1 < F.n[d] ? 2 * P[d] + 1 : 2 * P[d];
-                        } else {
-                            lo[d] = hi[d] = P[d];
-                        }
-                    }
-                    int64_t kd = 0, cp[3] = {0, 0, 0};
-                    for (int k = lo[2]; k <= hi[2]; ++k)
-                        for (int j = lo[1]; j <= hi[1]; ++j)
-                            for (int i = lo[0]; i <= hi[0]; ++i) {
-                                const int c[3] = {i, j, k};
-                                const int64_t fid = flat3(F.n, i, j, k);
-                                kd += K[(size_t)fid];
-                                for (int d = 0; d < D; ++d) {
-                                    // a coupling between two children of
-                                    // one parent: twice into the diagonal
-                                    if (c[d] < hi[d])
-                                        kd -= 2 * Cc[d][(size_t)fid];
-                                    else
-                                        cp[d] += Cc[d][(size_t)fid];
-                                }
-                            }
-                    const int64_t cid = flat3(C.n, I, J, Kz);
-                    Kc[(size_t)cid] = kd;
-                    for (int d = 0; d < D; ++d)
-                        Ccc[d][(size_t)cid] = cp[d];
-                }
-        // device coefficients
-        std::vector<double> hd((size_t)C.ncell), hw((size_t)C.ncell), hc((size_t)C.ncell);
-        auto upload = [&](const std::vector<double>& h) -> const double* {
-            std::unique_ptr<DevArray<double>> a(new DevArray<double>());
-            a->alloc(C.ncell, s);
-            CF_HIP(hipMemcpyAsync(a->ptr, h.data(), h.size() * sizeof(double),
-                                  hipMemcpyHostToDevice, s));
-            CF_HIP(hipStreamSynchronize(s));
-            const double* p = a->ptr;
-            _buf->coef.push_back(std::move(a));
-            return p;
-        };
-        for (int64_t i = 0; i < C.ncell; ++i) {
-            hd[(size_t)i] = (double)Kc[(size_t)i] * sc;
-            hw[(size_t)i] = omega / hd[(size_t)i];
+            nc *= C.n[d];
         }
-        C.diag = upload(hd);
-        C.w = upload(hw);
-        for (int d = 0; d < D; ++d) {
-            for (int64_t i = 0; i < C.ncell; ++i)
-                hc[(size_t)i] = -((double)Ccc[d][(size_t)i] * sc);
-            C.cp[d] = upload(hc);
-        }
-        K.swap(Kc);
-        for (int d = 0; d < D; ++d)
-            Cc[d].swap(Ccc[d]);
+        if (nc > INT_MAX)
+            throw CfError(CF_ENOTIMPL, "multigrid: a coarse level has more than 2^31 cells");
+        C.ncell = (int)nc;
         _lev.push_back(C);
     }
-    // work vectors
-    _buf->z[0].alloc(g.total, s);
-    _buf->z[1].alloc(g.total, s);
-    _buf->r0.alloc(g.total, s);
     const int nl = (int)_lev.size();
-    _buf->xcur.assign(nl, nullptr);
-    _buf->x.assign(nl, nullptr);
-    _buf->t.assign(nl, nullptr);
-    _buf->b.assign(nl, nullptr);
-    for (int l = 1; l < nl; ++l) {
-        for (double** dst : {&_buf->x[l], &_buf->t[l], &_buf->b[l]}) {
-            std::unique_ptr<DevArray<double>> a(new DevArray<double>());
-            a->alloc(_lev[l].ncell, s);
-            *dst = a->ptr;
-            _buf->vec.push_back(std::move(a));
+    const int last = nl - 1;
+    // the tail: the coarsest levels whose X, T, B fit the LDS budget
+    _tail0 = last;
+    {
+        int64_t cells = 0;
+        for (int l = last; l >= 1; --l) {
+            cells += _lev[l].ncell;
+            if (3 * cells * (int64_t)sizeof(double) > kMgTailLdsBytes)
+                break;
+            _tail0 = l;
         }
     }
+    // tiled levels
+    MgTab wd0;
     for (int k = 0; k < 7; ++k)
-        _buf->wd0.v[k] = omega * g.dinv[k];
+        wd0.v[k] = cfg.mg_omega * g.dinv[k];
+    _nt = 0;
+    if (last >= 1 && mg_tiled3(g)) {
+        Geo gw = g;
+        for (int k = 0; k < 7; ++k)
+            gw.dinv[k] = wd0.v[k];
+        _lg.push_back(g);
+        _lgw.push_back(gw);
+        _nt = 1;
+        for (int l = 1; l < _tail0; ++l) {
+            const MgLev& L = _lev[l];
+            bool ok = L.coarsen[0] && L.coarsen[1] && L.coarsen[2];
+            for (int d = 0; d < 3; ++d)
+                ok = ok && L.shift[d] == l && g.n[d] % (1 << l) == 0;
+            if (!ok)
+                break;
+            Geo lg, lgw;
+            tiled_level_geo(g, L, l, cfg.mg_omega, lg, lgw);
+            _lg.push_back(lg);
+            _lgw.push_back(lgw);
+            _nt = l + 1;
+        }
+    }
+    _wt.assign(nl, MgTab{});
+    _wt[0] = wd0;
+    for (int l = 1; l < _nt; ++l)
+        for (int k = 0; k < 7; ++k)
+            _wt[l].v[k] = _lgw[l].dinv[k];
+    _pm.assign(nl, MgParentMap{0, 0, 0});
+    for (int l = 0; l + 1 < nl; ++l)
+        _pm[l] = l + 1 < _nt ? padded_map(_lg[l + 1]) : compact_map(_lev[l + 1]);
+    // buffers
+    Buffers& B = *_buf;
+    B.z[0].alloc(g.total, s);
+    B.z[1].alloc(g.total, s);
+    B.X.assign(nl, nullptr);
+    B.T.assign(nl, nullptr);
+    B.B.assign(nl, nullptr);
+    B.res.assign(nl, nullptr);
+    B.X[0] = B.z[0].ptr;
+    B.T[0] = B.z[1].ptr;
+    for (int l = 1; l < nl; ++l) {
+        const int64_t len = l < _nt ? _lg[l].total : (int64_t)_lev[l].ncell;
+        for (double** dst : {&B.X[l], &B.T[l], &B.B[l]}) {
+            std::unique_ptr<DevArray<double>> a(new DevArray<double>());
+            a->alloc(len, s);
+            *dst = a->ptr;
+            B.vec.push_back(std::move(a));
+        }
+    }
+    for (int l = 0; l < nl; ++l)
+        B.res[l] = (mg_level_swaps(l == last, _nu, _coarse) & 1) ? B.T[l] : B.X[l];
+    if (last >= 1) {
+        MgTail h{};
+        h.l0 = _tail0;
+        h.last = last;
+        h.nu = _nu;
+        h.coarse = _coarse;
+        int off = 0;
+        for (int l = 0; l < nl; ++l) {
+            h.lev[l] = _lev[l];
+            h.B[l] = B.B[l];
+            h.res[l] = B.res[l];
+            h.off[l] = off;
+            if (l >= _tail0)
+                off += 3 * _lev[l].ncell;
+        }
+        _tail_lds = (size_t)off * sizeof(double);
+        void* p = nullptr;
+        CF_HIP(hipMalloc(&p, sizeof(MgTail)));
+        B.tail = (MgTail*)p;
+        CF_HIP(hipMemcpyAsync(B.tail, &h, sizeof(MgTail), hipMemcpyHostToDevice, s));
+    }
     CF_HIP(hipStreamSynchronize(s));
 }
 
 Multigrid::~Multigrid() = default;
 
-// V-cycle on level l >= 1 with its right-hand side in b[l]; x ends in xcur[l].
-void Multigrid::cycle(int l, const double* b, const PcgState* st)
+// x = w b on level l
+void Multigrid::first(int l, const double* b, double* x, PcgState* st)
 {
-    Buffers& B = *_buf;
-    const int last = (int)_lev.size() - 1;
-    const MgLev& L = _lev[l];
-    double* x = B.x[l];
-    double* t = B.t[l];
-    const int pre = l == last ? _coarse : _nu;
-    launch_mg_first(L, b, x, st, _s);
-    for (int it = 1; it < pre; ++it) {
-        launch_mg_sweep(L, x, b, t, st, _s);
-        std::swap(x, t);
-    }
-    if (l < last) {
-        launch_mg_residual_restrict(L, _lev[l + 1], x, b, B.b[l + 1], st, _s);
-        cycle(l + 1, B.b[l + 1], st);
-        launch_mg_prolong(L, _lev[l + 1], B.xcur[l + 1], x, st, _s);
-        for (int it = 0; it < _nu; ++it) {
-            launch_mg_sweep(L, x, b, t, st, _s);
-            std::swap(x, t);
-        }
-    }
-    B.xcur[l] = x;
-    // keep the two buffers distinct for the next cycle
-    B.x[l] = x;
-    B.t[l] = t;
+    if (l == 0 || l < _nt)
+        launch_mg0_first(l == 0 ? _g : _lg[l], _wt[l], b, x, st, _s);
+    else
+        launch_mg_first(_lev[l], b, x, st, _s);
 }
 
-const double* Multigrid::vcycle(const double* r, const PcgState* st)
+// one sweep on level l (l below the coarsest); the prolongation source is
+// level l + 1's cycle result
+void Multigrid::smooth(int l, int src, bool rz, const double* xi, const double* b, double* xo,
+                       PcgState* st, RedBuf rb)
+{
+    const double* xc = _buf->res[l + 1];
+    if (l < _nt)
+        launch_mg_smooth3(_lg[l], _lgw[l], src, rz, xi, b, xc, _pm[l], _lev[0].alpha, xo, st, rb,
+                          _s);
+    else if (l == 0)
+        launch_mg0_smooth(_g, _wt[0], src, rz, xi, b, _lev[1], xc, xo, st, rb, _s);
+    else
+        launch_mg_smooth(_lev[l], src, xi, b, _lev[l + 1], xc, xo, st, _s);
+}
+
+// level l + 1's right-hand side from level l
+void Multigrid::resrestrict(int l, const double* x, const double* b, PcgState* st)
+{
+    double* bc = _buf->B[l + 1];
+    if (l < _nt)
+        launch_mg_resrestrict3(_lg[l], x, b, bc, _pm[l], st, _s);
+    else if (l == 0)
+        launch_mg0_resrestrict(_g, _lev[1], x, b, bc, st, _s);
+    else
+        launch_mg_resrestrict(_lev[l], _lev[l + 1], x, b, bc, st, _s);
+}
+
+const double* Multigrid::vcycle(const double* r, PcgState* st, RedBuf rb)
 {
     Buffers& B = *_buf;
     const int last = (int)_lev.size() - 1;
-    double* x = B.z[0].ptr;
-    double* t = B.z[1].ptr;
-    const int pre = last == 0 ? _coarse : _nu;
-    launch_mg0_first(_g, B.wd0, r, x, st, _s);
-    for (int it = 1; it < pre; ++it) {
-        launch_mg0_sweep(_g, B.wd0, x, r, t, st, _s);
-        std::swap(x, t);
+    if (last == 0) {
+        // level 0 is the coarsest: coarse sweeps from zero
+        double* x = B.X[0];
+        double* t = B.T[0];
+        if (_coarse == 1)
+            first(0, r, x, st);
+        else
+            launch_mg0_smooth(_g, _wt[0], MG_FIRST, false, nullptr, r, _lev[0], nullptr, x, st,
+                              rb, _s);
+        for (int it = 2; it < _coarse; ++it) {
+            launch_mg0_smooth(_g, _wt[0], MG_PLAIN, false, x, r, _lev[0], nullptr, t, st, rb,
+                              _s);
+            std::swap(x, t);
+        }
+        launch_mgpcg_rz(_g, r, x, st, rb, _s);
+        return x;
     }
-    if (last > 0) {
-        launch_mg0_residual(_g, x, r, B.r0.ptr, st, _s);
-        launch_mg0_restrict(_g, _lev[1], B.r0.ptr, B.b[1], st, _s);
-        cycle(1, B.b[1], st);
-        launch_mg0_prolong(_g, _lev[1], B.xcur[1], x, st, _s);
-        for (int it = 0; it < _nu; ++it) {
-            launch_mg0_sweep(_g, B.wd0, x, r, t, st, _s);
+    // down: pre-smoothing from zero and restriction
+    for (int l = 0; l < _tail0; ++l) {
+        const double* b = l == 0 ? r : B.B[l];
+        double* x = B.X[l];
+        double* t = B.T[l];
+        if (_nu == 1)
+            first(l, b, x, st);
+        else
+            smooth(l, MG_FIRST, false, nullptr, b, x, st, rb);
+        for (int it = 2; it < _nu; ++it) {
+            smooth(l, MG_PLAIN, false, x, b, t, st, rb);
+            std::swap(x, t);
+        }
+        resrestrict(l, x, b, st);
+    }
+    launch_mg_tail(B.tail, _tail_lds, st, _s);
+    // up: the first post-sweep reads the coarse correction; on level 0 the
+    // last sweep also reduces r.z
+    const bool pre_odd = (_nu >= 2 ? _nu - 2 : 0) & 1;
+    double* x = nullptr;
+    for (int l = _tail0 - 1; l >= 0; --l) {
+        const double* b = l == 0 ? r : B.B[l];
+        x = pre_odd ? B.T[l] : B.X[l];
+        double* t = pre_odd ? B.X[l] : B.T[l];
+        smooth(l, MG_PROLONG, l == 0 && _nu == 1, x, b, t, st, rb);
+        std::swap(x, t);
+        for (int it = 1; it < _nu; ++it) {
+            smooth(l, MG_PLAIN, l == 0 && it == _nu - 1, x, b, t, st, rb);
             std::swap(x, t);
         }
     }

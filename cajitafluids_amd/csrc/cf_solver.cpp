@@ -452,7 +452,7 @@ cf_status VelocityCorrector::solve(int precision, const double* b, double* x,
 // MG-PCG: Jacobi-PCG's kernels with z = B r from one V-cycle.  The
 // direction kernel runs on a geometry whose Jacobi table is 1, so it forms
 // p = fma(beta, p_old, z) (z alone on the first iteration); the update
-// kernel leaves beta and rho to k_mgpcg_rz, which follows the V-cycle.
+// kernel leaves beta and rho to the V-cycle's last sweep, which reduces r.z.
 // Iterations are queued in batches like runIterations; every kernel of an
 // iteration queued after convergence returns at once.
 cf_status VelocityCorrector::solveMg(const double* b, double* x, cf_solve_stats* st)
@@ -461,8 +461,7 @@ cf_status VelocityCorrector::solveMg(const double* b, double* x, cf_solve_stats*
         _mg.reset(new Multigrid(_g, _cfg, _s));
     RedBuf rb{_red_partials.ptr, _red_counter.ptr};
     launch_mgpcg_init(_g, b, x, _r.ptr, _st, _cfg.cg_tol, _cfg.cg_max_iter, rb, _s);
-    const double* z = _mg->vcycle(_r.ptr, _st);
-    launch_mgpcg_rz(_g, _r.ptr, z, _st, rb, _s);
+    const double* z = _mg->vcycle(_r.ptr, _st, rb);
     PcgState hs;
     readState(hs);
     int cur = 0, launched = 0;
@@ -478,9 +477,8 @@ cf_status VelocityCorrector::solveMg(const double* b, double* x, cf_solve_stats*
             _timer.end(t1, _s, TK_UPDATE, it);
             // the V-cycle prepares iteration it + 1 (a no-op after the stop)
             int t2 = _timer.begin_iter(_s, it + 1);
-            z = _mg->vcycle(_r.ptr, _st);
+            z = _mg->vcycle(_r.ptr, _st, rb);
             _timer.end(t2, _s, TK_VCYCLE, it + 1);
-            launch_mgpcg_rz(_g, _r.ptr, z, _st, rb, _s);
             cur ^= 1;
         }
         launched += batch;

@@ -18,6 +18,11 @@
 // block, everything that depends on a row once per block and wave (rows are
 // wave-uniform), and everything that depends on the plane once per plane, so
 // the per-cell work is the arithmetic itself.
+//
+// The value marched through the stencil is chosen by MODE: the loaded
+// vector (0), the PCG direction fma(beta, p_old, D^-1 r) (1), or the loaded
+// vector plus beta times its parent's value on a grid coarsened by 2 along
+// every axis (2, the multigrid prolongation; CPL = 1).
 #pragma once
 
 #include "cf_stream.hpp"
@@ -45,6 +50,28 @@ constexpr int tx()
 struct Tile3Grid
 {
     int tiles_x, tiles_y, nchunk, kz, work;
+};
+
+// Parent grid of MODE 2: value(i, j, k) = fma(alpha, xc[base + i/2 + sy j/2 +
+// sz k/2], a(i, j, k)) (the parent array compact or padded); the march
+// receives alpha as its `beta`.
+struct Tile3Parent
+{
+    const double* xc;
+    int64_t base, sy, sz;
+    double alpha;
+};
+
+// An epilogue that declares `static constexpr bool kRaw = true` receives the
+// raw loaded operand of its own rows (epi.take_raw(plane)) just before each
+// prefetch of the same plane.
+template <class E, class = void>
+struct epi_takes_raw : std::false_type
+{
+};
+template <class E>
+struct epi_takes_raw<E, std::void_t<decltype(E::kRaw)>> : std::bool_constant<E::kRaw>
+{
 };
 
 template <int CPL>
@@ -182,6 +209,7 @@ struct Tile3Ctx
     int i0, j0, k0, k1;
     int lane, w;
     int64_t colb[tile3::NS];
+    int sgi[tile3::NS], sgj[tile3::NS]; // (wrapped) x, y of each slot's first cell
     // a group whose last cell is the row's end, on a periodic x axis: its
     // second cell (one past the end) is the row's first cell (CPL = 2)
     bool split, hsplit;
@@ -236,6 +264,8 @@ __device__ __forceinline__ Tile3Ctx<CPL> tile3_ctx(const Geo& g, const Tile3Grid
         // cells just past the domain stand in for their periodic images
         // (or zero ghosts) as neighbours of the last own cells
         c.colb[q] = col(gi0, gj);
+        c.sgi[q] = wrap_or_ghost(g, 0, gi0);
+        c.sgj[q] = b;
         c.wy[q] = b < 0 ? 0 : wall_count(g, 1, b);
         c.ywall_m[q] = wall_lo(g, 1, gj);
         c.ywall_p[q] = wall_hi(g, 1, gj);
@@ -252,6 +282,7 @@ __device__ __forceinline__ Tile3Ctx<CPL> tile3_ctx(const Geo& g, const Tile3Grid
     c.hrow = -1;
     c.hcol = 0;
     c.colb[RY] = -1;
+    c.sgi[RY] = c.sgj[RY] = 0;
 #pragma unroll
     for (int j = 0; j < CPL; ++j)
         c.hwxy[j] = 0;
@@ -264,6 +295,8 @@ __device__ __forceinline__ Tile3Ctx<CPL> tile3_ctx(const Geo& g, const Tile3Grid
         c.hrow = hh < LX ? 0 : TY + 1;
         c.hcol = l2 * CPL + 1;
         c.colb[RY] = col(gi, gj);
+        c.sgi[RY] = wrap_or_ghost(g, 0, gi);
+        c.sgj[RY] = wrap_or_ghost(g, 1, gj);
         c.hsplit = CPL > 1 && g.periodic[0] && gi + 1 == g.n[0];
         const int b = wrap_or_ghost(g, 1, gj);
 #pragma unroll
@@ -281,6 +314,8 @@ __device__ __forceinline__ Tile3Ctx<CPL> tile3_ctx(const Geo& g, const Tile3Grid
         c.hrow = r + 1;
         c.hcol = left ? 0 : tx<CPL>() + 1;
         c.colb[RY] = col(gi, gj);
+        c.sgi[RY] = wrap_or_ghost(g, 0, gi);
+        c.sgj[RY] = wrap_or_ghost(g, 1, gj);
         if (c.colb[RY] >= 0)
             c.hwxy[0] = wall_count(g, 0, wrap_or_ghost(g, 0, gi)) +
                         wall_count(g, 1, wrap_or_ghost(g, 1, gj));
@@ -294,6 +329,7 @@ struct Plane3Meta
 {
     int64_t off;
     int wz;
+    int zc; // wrapped z, or -1
     bool zwall_m, zwall_p;
 };
 
@@ -302,6 +338,7 @@ __device__ __forceinline__ Plane3Meta plane3_meta(const Geo& g, int gk)
     Plane3Meta m;
     const int zc = wrap_or_ghost(g, 2, gk);
     m.off = zc < 0 ? -1 : g.px * g.py * (int64_t)(g.h + zc);
+    m.zc = zc;
     m.wz = zc < 0 ? 0 : wall_count(g, 2, zc);
     m.zwall_m = wall_lo(g, 2, gk);
     m.zwall_p = wall_hi(g, 2, gk);
@@ -309,20 +346,22 @@ __device__ __forceinline__ Plane3Meta plane3_meta(const Geo& g, int gk)
 }
 
 // Raw operands of one plane: a = r (direction) or p (update), b = old
-// direction (direction kernel only), all 0 at zero ghosts.
-template <class T, bool DIR, int CPL>
+// direction (MODE 1) or the parent's value (MODE 2), all 0 at zero ghosts.
+template <class T, int MODE, int CPL>
 struct Plane3
 {
     T a[tile3::NS][CPL];
     T b[tile3::NS][CPL];
 };
 
-template <class T, bool DIR, int CPL>
+template <class T, int MODE, int CPL>
 __device__ __forceinline__ void tile3_load(const Tile3Ctx<CPL>& c, const Plane3Meta& m,
                                            const T* __restrict__ va,
                                            const T* __restrict__ vb, bool need_b,
-                                           Plane3<T, DIR, CPL>& P)
+                                           const Tile3Parent& par, Plane3<T, MODE, CPL>& P)
 {
+    constexpr bool DIR = MODE == 1;
+    static_assert(MODE != 2 || CPL == 1, "the prolongation mode runs one cell per lane");
 #pragma unroll
     for (int s = 0; s < tile3::NS; ++s) {
         const bool ok = c.colb[s] >= 0 && m.off >= 0;
@@ -337,6 +376,15 @@ __device__ __forceinline__ void tile3_load(const Tile3Ctx<CPL>& c, const Plane3M
         // lane's group of CPL cells, read as one access unless it wraps
         const bool single = CPL > 1 && s == tile3::RY && c.hkind == 2;
         const bool split = CPL > 1 && (s < tile3::RY ? c.split : c.hsplit);
+        if constexpr (MODE == 2) {
+            P.b[s][0] = (T)0;
+            if (ok) {
+                P.a[s][0] = va[at];
+                P.b[s][0] = par.xc[par.base + (c.sgi[s] >> 1) + par.sy * (c.sgj[s] >> 1) +
+                                   par.sz * (m.zc >> 1)];
+            }
+            continue;
+        }
         if (ok && !single && !split) {
             vload<T, CPL>(va + at, P.a[s]);
             if constexpr (DIR)
@@ -360,15 +408,18 @@ __device__ __forceinline__ void tile3_load(const Tile3Ctx<CPL>& c, const Plane3M
 }
 
 // Value of cell j of slot s: the new direction fma(beta, p_old, D^-1 r)
-// (D^-1 r alone on the first iteration) for DIR, else the loaded p.
-template <class T, bool DIR, int CPL>
+// (D^-1 r alone on the first iteration) for MODE 1, fma(beta, parent, a) for MODE 2,
+// else the loaded vector.
+template <class T, int MODE, int CPL>
 __device__ __forceinline__ T tile3_value(const Geo& g, const Tile3Ctx<CPL>& c,
                                          const Plane3Meta& m,
-                                         const Plane3<T, DIR, CPL>& P, int s, int j,
+                                         const Plane3<T, MODE, CPL>& P, int s, int j,
                                          bool first, T beta)
 {
-    if constexpr (!DIR) {
+    if constexpr (MODE == 0) {
         return P.a[s][j];
+    } else if constexpr (MODE == 2) {
+        return fma_t<T>(beta, P.b[s][j], P.a[s][j]);
     } else {
         T dinv;
         if (s < tile3::RY)
@@ -382,7 +433,7 @@ __device__ __forceinline__ T tile3_value(const Geo& g, const Tile3Ctx<CPL>& c,
 
 // Writes the plane's values into LDS plane pl and own[]; the halo slot
 // fills its ring cells.
-template <class T, bool DIR, int CPL>
+template <class T, int DIR, int CPL>
 __device__ __forceinline__ void tile3_finish(const Geo& g, const Tile3Ctx<CPL>& c,
                                              const Plane3Meta& m,
                                              const Plane3<T, DIR, CPL>& P, bool first,
@@ -438,34 +489,38 @@ __device__ __forceinline__ T tile3_apply(const Geo& g, const Tile3Ctx<CPL>& c,
     return y;
 }
 
-// The z march shared by the direction (DIR) and update kernels;
-// epi.cells() receives each own row's centre values and A p.
-template <class T, bool DIR, int CPL, class Epi>
+// The z march shared by the direction (MODE 1), update (MODE 0) and
+// multigrid kernels; epi.cells() receives each own row's centre values and
+// A applied to the values.
+template <class T, int DIR, int CPL, class Epi>
 __device__ __forceinline__ void tile3_march(
     const Geo& g, const Tile3Grid& tg, const T* __restrict__ va,
     const T* __restrict__ vb, bool first, T beta,
-    T (*pl)[tile3::TY + 2][tile3::tx<CPL>() + 2], Epi&& epi)
+    T (*pl)[tile3::TY + 2][tile3::tx<CPL>() + 2], Epi&& epi,
+    const Tile3Parent par = Tile3Parent{nullptr, 0, 0, 0, 0.0})
 {
     using namespace tile3;
     const Tile3Ctx<CPL> c = tile3_ctx<CPL>(g, tg);
     if (!c.active)
         return;
-    const bool need_b = !first;
+    const bool need_b = DIR == 2 || !first;
     Plane3<T, DIR, CPL> P0, P1;
     T pm[RY][CPL], pc[RY][CPL], pp[RY][CPL];
     // prologue: plane k0-1 (own column only), plane k0 into LDS, k0+1 in flight
     Plane3Meta mm = plane3_meta(g, c.k0 - 1);
-    tile3_load<T, DIR, CPL>(c, mm, va, vb, need_b, P0);
+    tile3_load<T, DIR, CPL>(c, mm, va, vb, need_b, par, P0);
 #pragma unroll
     for (int q = 0; q < RY; ++q)
 #pragma unroll
         for (int j = 0; j < CPL; ++j)
             pm[q][j] = tile3_value<T, DIR, CPL>(g, c, mm, P0, q, j, first, beta);
     Plane3Meta mc = plane3_meta(g, c.k0);
-    tile3_load<T, DIR, CPL>(c, mc, va, vb, need_b, P1);
+    tile3_load<T, DIR, CPL>(c, mc, va, vb, need_b, par, P1);
     tile3_finish<T, DIR, CPL>(g, c, mc, P1, first, beta, pl[0], pc);
     Plane3Meta mn = plane3_meta(g, c.k0 + 1);
-    tile3_load<T, DIR, CPL>(c, mn, va, vb, need_b, P0);
+    tile3_load<T, DIR, CPL>(c, mn, va, vb, need_b, par, P0);
+    if constexpr (epi_takes_raw<std::decay_t<Epi>>::value)
+        epi.take_raw(P1);
     epi.prefetch(c, mc, c.k0);
     epi.rotate();
     __syncthreads();
@@ -474,8 +529,10 @@ __device__ __forceinline__ void tile3_march(
         // P0 holds plane k+1 (loaded last step); start plane k+2
         const Plane3Meta mnn = plane3_meta(g, k + 2);
         if (k + 2 <= c.k1)
-            tile3_load<T, DIR, CPL>(c, mnn, va, vb, need_b, P1);
+            tile3_load<T, DIR, CPL>(c, mnn, va, vb, need_b, par, P1);
         tile3_finish<T, DIR, CPL>(g, c, mn, P0, first, beta, pl[cur ^ 1], pp);
+        if constexpr (epi_takes_raw<std::decay_t<Epi>>::value)
+            epi.take_raw(P0);
         epi.prefetch(c, mn, k + 1);
 #pragma unroll
         for (int q = 0; q < RY; ++q) {

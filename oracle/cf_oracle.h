@@ -89,6 +89,7 @@ typedef struct cfo_config {
     int mg_nu;            /* Jacobi sweeps before and after the coarse correction */
     double mg_omega;      /* Jacobi damping */
     int mg_coarse_sweeps; /* Jacobi sweeps on the coarsest level */
+    double mg_alpha;      /* scale of the coarse-grid correction */
 } cfo_config;
 
 enum { CFO_REFINE_RESTART = 0, CFO_REFINE_CARRY = 1 };

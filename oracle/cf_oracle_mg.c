@@ -23,10 +23,12 @@
  *              (omega * (1 / diag) on level 0, the PCG's Jacobi table); the
  *              first sweep from x = 0 is w b
  *   V-cycle    nu sweeps, residual restricted by summing children (z, y, x
- *              order), recursion, prolongation by adding the parent's value,
+ *              order), recursion, prolongation x = fma(alpha, x_parent, x),
  *              nu sweeps; the coarsest level takes coarse_sweeps sweeps from
  *              zero.  Equal pre and post sweeps of a symmetric smoother make
- *              the cycle a symmetric preconditioner
+ *              the cycle a symmetric preconditioner.  alpha > 1 over-corrects
+ *              (Braess's remedy for the piecewise-constant P, whose coarse
+ *              correction alone is too small by about half in energy)
  *   MG-PCG     as Jacobi-PCG with z = V(r) (cfo_pcg_mg)
  *
  * A-x order everywhere: diag x, then fma over -x, +x, -y, +y, -z, +z.
@@ -294,7 +296,7 @@ static void mg_restrict(const struct cfo_state* s, int f, const double* r, doubl
             }
 }
 
-/* x_f += x_c[parent] */
+/* x_f = fma(alpha, x_c[parent], x_f) */
 static void mg_prolong(const struct cfo_state* s, int f, const double* xc, double* xf)
 {
     const cfo_mglev* F = &s->mg->lev[f];
@@ -307,7 +309,7 @@ static void mg_prolong(const struct cfo_state* s, int f, const double* xc, doubl
                 for (int d = 0; d < 3; ++d)
                     P[d] = (d < D && F->coarsen[d]) ? c[d] / 2 : c[d];
                 long fi = f == 0 ? g0(s, i, j, k) : lidx(F, i, j, k);
-                xf[fi] = xf[fi] + xc[lidx(C, P[0], P[1], P[2])];
+                xf[fi] = fma(s->c.mg_alpha, xc[lidx(C, P[0], P[1], P[2])], xf[fi]);
             }
 }
 

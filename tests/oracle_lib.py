@@ -57,6 +57,7 @@ class CfoConfig(C.Structure):
         ("mg_nu", C.c_int),
         ("mg_omega", C.c_double),
         ("mg_coarse_sweeps", C.c_int),
+        ("mg_alpha", C.c_double),
     ]
 
 

@@ -49,6 +49,15 @@ CASES = {
     "3d_free_top_24x20x17": _cfg(3, (24, 20, 17),
                                  walls=[[SOLID, SOLID], [SOLID, SOLID], [SOLID, FREE]]),
     "3d_nu1_coarse4_20": _cfg(3, (20, 20, 20), mg_nu=1, mg_coarse_sweeps=4, mg_omega=0.6),
+    "3d_alpha1_nu3_24": _cfg(3, (24, 24, 24), mg_alpha=1.0, mg_nu=3),
+    "2d_alpha1_40": _cfg(2, (40, 40), mg_alpha=1.0),
+    # level 1 as a compact (non-tail) level
+    "3d_odd_45": _cfg(3, (45, 45, 45)),
+    "2d_200x150": _cfg(2, (200, 150)),
+    # tiled coarse levels (full 2^l boxes): level 1 padded, periodic / FREE
+    "3d_periodic_xz_48x32x40": _cfg(3, (48, 32, 40), periodic=(1, 0, 1)),
+    "3d_free_top_64": _cfg(3, (64, 64, 64),
+                           walls=[[SOLID, SOLID], [SOLID, SOLID], [SOLID, FREE]]),
 }
 
 
@@ -82,7 +91,9 @@ def test_mg_pcg_matches_oracle(cf, oracle_mod, name):
 
 @pytest.mark.parametrize("maxit", [1, 2, 5])
 @pytest.mark.parametrize("name", ["2d_free_45x27", "3d_ragged_70x36x40",
-                                  "3d_periodic_xy_21x18x11", "3d_nu1_coarse4_20"])
+                                  "3d_periodic_xy_21x18x11", "3d_nu1_coarse4_20",
+                                  "3d_alpha1_nu3_24", "3d_odd_45",
+                                  "3d_periodic_xz_48x32x40"])
 def test_mg_pcg_truncated(cf, oracle_mod, name, maxit):
     from cajitafluids_amd import _lib
     cfg = K.copy_config(CASES[name])

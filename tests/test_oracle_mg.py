@@ -96,7 +96,7 @@ def _vcycle_dense(o, cfg, nlev, dims, A):
         if l == nlev - 1:
             return x
         xc = cycle(l + 1, P[l].T @ (b - A[l] @ x))
-        x = x + P[l] @ xc
+        x = x + cfg["mg_alpha"] * (P[l] @ xc)
         for _ in range(nu):
             x = x + W[l] * (b - A[l] @ x)
         return x
@@ -165,3 +165,68 @@ def test_mg_cuts_iterations(oracle_mod):
         its[pre] = st["iterations"]
         o.close()
     assert its[K.PRECOND_MG] * 5 < its[K.PRECOND_JACOBI], its
+
+
+def _closed_form_operator(cfg, dims, l, s):
+    """A coarse level's operator from the boxes of level-0 cells it
+    aggregates: the coupling to a neighbouring box along d is -(area_d s),
+    area_d the product of the box's extents along the other axes; the
+    diagonal is K s, K the box's boundary faces that are not SOLID walls.
+    This is the formula the HIP kernels evaluate instead of stored
+    coefficients."""
+    D = cfg["dim"]
+    per = [bool(cfg["periodic"][d]) for d in range(3)]
+    solid = [[cfg["wall_type"][d][e] == SOLID for e in range(2)] for d in range(3)]
+    n0 = dims[0]
+    shift = [0, 0, 0]
+    for m in range(l):
+        for d in range(D):
+            if dims[m][d] > 2:
+                shift[d] += 1
+    n = dims[l]
+    N = int(np.prod(n))
+    A = np.zeros((N, N))
+
+    def ext(d, P):
+        lo = P << shift[d]
+        return min(lo + (1 << shift[d]), n0[d]) - lo
+
+    for k in range(n[2]):
+        for j in range(n[1]):
+            for i in range(n[0]):
+                c = (i, j, k)
+                row = i + n[0] * (j + n[1] * k)
+                e = [ext(d, c[d]) if d < D else 1 for d in range(3)]
+                K = 0
+                for d in range(D):
+                    area = int(np.prod([e[f] for f in range(3) if f != d]))
+                    lo_cut = not per[d] and c[d] == 0 and solid[d][0]
+                    hi_cut = not per[d] and c[d] == n[d] - 1 and solid[d][1]
+                    K += (0 if lo_cut else area) + (0 if hi_cut else area)
+                    for dirn in (-1, 1):
+                        m = list(c)
+                        m[d] += dirn
+                        if m[d] < 0 or m[d] >= n[d]:
+                            if not per[d]:
+                                continue
+                            m[d] %= n[d]
+                        col = m[0] + n[0] * (m[1] + n[1] * m[2])
+                        A[row, col] = A[row, col] + (-(float(area) * s))
+                A[row, row] = A[row, row] + float(K) * s
+    return A
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_coarse_operators_closed_form(oracle_mod, name):
+    """Every coarse operator the oracle assembles by counting faces level by
+    level equals, entry for entry and bit for bit, the closed form over
+    boxes of level-0 cells."""
+    cfg = CASES[name]
+    o = oracle_mod.Oracle(cfg)
+    nlev = o.mg_nlevels()
+    dims = [o.mg_level_dims(l) for l in range(nlev)]
+    s = o.dt / (cfg["density"] * o.dx * o.dx)
+    for l in range(1, nlev):
+        A = _dense(lambda x, l=l: o.mg_apply(l, x), int(np.prod(dims[l])))
+        np.testing.assert_array_equal(A, _closed_form_operator(cfg, dims, l, s))
+    o.close()
