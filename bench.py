@@ -56,6 +56,9 @@ def parse():
     ap.add_argument("--summarize-counters", metavar="CSV",
                     help="print per-kernel dispatch count, median and sum of a "
                          "rocprofv3 --pmc counter CSV, then exit (no GPU use)")
+    ap.add_argument("--summarize-trace", metavar="CSV",
+                    help="print per-kernel, per-grid-size dispatch count and mean / total "
+                         "duration of a rocprofv3 --kernel-trace CSV, then exit (no GPU use)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a plain 1 GiB device copy, the reference "
                          "kernel for scaling rocprof FETCH_SIZE / WRITE_SIZE")
@@ -108,6 +111,34 @@ def _lib_tk_count():
     return _lib.TK_VCYCLE + 1
 
 
+def summarize_trace(path):
+    """Per (kernel, grid size): dispatches, mean and total duration of a
+    rocprofv3 kernel_trace CSV, largest total first, and the means of the
+    even- and odd-numbered dispatches in launch order.  The grid size tells
+    the multigrid levels apart; where two levels share a grid size and
+    alternate, the even / odd means separate them."""
+    import csv
+    vals = {}
+    with open(path) as f:
+        rows = sorted(csv.DictReader(f), key=lambda r: int(r["Start_Timestamp"]))
+        for r in rows:
+            gsz = 1
+            for ax in ("X", "Y", "Z"):
+                gsz *= int(r.get(f"Grid_Size_{ax}", 1) or 1)
+            if "Grid_Size" in r:
+                gsz = int(r["Grid_Size"])
+            key = (r["Kernel_Name"], gsz)
+            vals.setdefault(key, []).append(
+                (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3)
+    w = csv.writer(sys.stdout)
+    w.writerow(["kernel", "grid_items", "dispatches", "mean_us", "total_ms", "mean_even_us",
+                "mean_odd_us"])
+    for (k, gsz), v in sorted(vals.items(), key=lambda kv: -sum(kv[1])):
+        ev, od = v[0::2], v[1::2]
+        w.writerow([k[:80], gsz, len(v), round(sum(v) / len(v), 2), round(sum(v) * 1e-3, 3),
+                    round(sum(ev) / len(ev), 2), round(sum(od) / len(od), 2) if od else ""])
+
+
 def summarize_counters(path):
     """Per-kernel dispatch count, median and sum of each counter in a
     rocprofv3 counter_collection CSV, largest sum first."""
@@ -128,6 +159,9 @@ def main():
     args = parse()
     if args.summarize_counters:
         summarize_counters(args.summarize_counters)
+        return
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the "

@@ -9,7 +9,7 @@ from .configs import (CELL, FACE_X, FACE_Y, FACE_Z, FP32_REFINED, FP64,  # noqa:
 def __getattr__(name):
     # Loading the solver module is deferred so that `import cajitafluids_amd`
     # works for configs alone; the library itself is loaded on first use.
-    if name in ("Solver", "create_solver", "BoundaryCondition", "InflowSource",
+    if name in ("Solver", "create_solver", "solver_config", "BoundaryCondition", "InflowSource",
                 "BodyForce", "layout", "geometry", "CURRENT", "NEXT"):
         from . import solver
         return getattr(solver, name)

@@ -528,6 +528,59 @@ __global__ __launch_bounds__(256) void k_mg_resrestrict(MgLev F, MgLev C,
     CF_GRID_STRIDE(id, C.ncell) { mg_resrestrict_cell(F, C, id, x, b, bc); }
 }
 
+// The coarsest level (every axis at most 2 cells, so at most 8 cells) by
+// one wave: lane i holds cell i's x and exchanges neighbour values by
+// shuffles, so its sweeps need no barrier.  Same arithmetic as
+// mg_first_cell / mg_smooth_cell (a neighbour's w b is the value that lane
+// computes itself).  The result goes to out.
+__device__ __forceinline__ void mg_coarsest_wave(const MgLev& L, const double* __restrict__ b,
+                                                 double* __restrict__ out, int sweeps)
+{
+    const int lane = threadIdx.x;
+    const bool on = lane < L.ncell;
+    int c[3];
+    unflatten(on ? lane : 0, L.n, c);
+    int64_t area[3], K;
+    lev_counts(L, c, area, K);
+    const double diag = (double)K * L.s;
+    const double w = L.omega / diag;
+    double cm[3] = {0.0, 0.0, 0.0}, cp[3] = {0.0, 0.0, 0.0};
+    int lm[3] = {lane, lane, lane}, lp[3] = {lane, lane, lane};
+    bool hm[3] = {false, false, false}, hp[3] = {false, false, false};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (d >= L.dim)
+            break;
+        const double ca = -((double)area[d] * L.s);
+        int m[3];
+        hm[d] = lev_nbr(L, c, d, -1, m);
+        if (hm[d])
+            lm[d] = flat(L.n, m[0], m[1], m[2]);
+        hp[d] = lev_nbr(L, c, d, +1, m);
+        if (hp[d])
+            lp[d] = flat(L.n, m[0], m[1], m[2]);
+        cm[d] = hm[d] ? ca : 0.0;
+        cp[d] = hp[d] ? ca : 0.0;
+    }
+    const double bv = on ? b[lane] : 0.0;
+    double x = w * bv;
+    for (int it = 1; it < sweeps; ++it) {
+        double acc = diag * x;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (d >= L.dim)
+                break;
+            const double xm = __shfl(x, lm[d]);
+            const double xp = __shfl(x, lp[d]);
+            acc = fma(cm[d], hm[d] ? xm : 0.0, acc);
+            acc = fma(cp[d], hp[d] ? xp : 0.0, acc);
+        }
+        x = fma(w, bv - acc, x);
+    }
+    if (on)
+        out[lane] = x;
+}
+
 // ---- the small levels in one workgroup ------------------------------------------------
 // Same cycle as the host runs on the large levels (Multigrid::vcycle), one
 // phase per sweep, __syncthreads between phases.  Every vector of these
@@ -553,14 +606,20 @@ __global__ __launch_bounds__(kMgTailThreads) void k_mg_tail(const MgTail* __rest
         CF_BLOCK_STRIDE(id, T->lev[l0].ncell) { b[id] = bg[id]; }
         __syncthreads();
     }
-    // down: pre-smoothing and restriction
+    // down: pre-smoothing and restriction; the coarsest level in wave 0
     for (int l = l0; l <= last; ++l) {
         const MgLev L = T->lev[l];
         const MgLev& C = L; // unused parent level of the non-prolong sweeps
         const double* b = Bv(l);
+        if (l == last) {
+            if (threadIdx.x < kWave)
+                mg_coarsest_wave(L, b, res(l), coarse);
+            __syncthreads();
+            break;
+        }
         double* x = X(l);
         double* t = Tv(l);
-        const int pre = l == last ? coarse : nu;
+        const int pre = nu;
         if (pre == 1) {
             CF_BLOCK_STRIDE(id, L.ncell) { mg_first_cell(L, id, b, x); }
         } else {
@@ -574,12 +633,10 @@ __global__ __launch_bounds__(kMgTailThreads) void k_mg_tail(const MgTail* __rest
             x = t;
             t = tmp;
         }
-        if (l < last) {
-            const MgLev Cn = T->lev[l + 1];
-            double* bn = Bv(l + 1);
-            CF_BLOCK_STRIDE(id, Cn.ncell) { mg_resrestrict_cell(L, Cn, id, x, b, bn); }
-            __syncthreads();
-        }
+        const MgLev Cn = T->lev[l + 1];
+        double* bn = Bv(l + 1);
+        CF_BLOCK_STRIDE(id, Cn.ncell) { mg_resrestrict_cell(L, Cn, id, x, b, bn); }
+        __syncthreads();
     }
     // up: prolongation fused into the first post-sweep
     for (int l = last - 1; l >= l0; --l) {

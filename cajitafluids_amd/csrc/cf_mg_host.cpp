@@ -130,6 +130,8 @@ Multigrid::Multigrid(const Geo& g, const cf_config& cfg, hipStream_t s)
     }
     const int nl = (int)_lev.size();
     const int last = nl - 1;
+    if (_lev[last].ncell > 64)
+        throw CfError(CF_ENOTIMPL, "multigrid: the coarsest level must fit one wave");
     // the tail: the coarsest levels whose X, T, B fit the LDS budget
     _tail0 = last;
     {

@@ -14,8 +14,8 @@ import numpy as np
 
 from . import _lib
 from .configs import (CELL, FACE_X, FACE_Y, FACE_Z, FP32_REFINED, FP64,
-                      INFLOW_BOX, INFLOW_NONE, QUIRK_PROJ_BC_U, QUIRK_RK3_V0,
-                      SOLID, base_config)
+                      INFLOW_BOX, INFLOW_NONE, PRECOND_JACOBI, PRECOND_MG,
+                      QUIRK_PROJ_BC_U, QUIRK_RK3_V0, SOLID, base_config)
 
 CURRENT, NEXT = 0, 1
 
@@ -279,14 +279,37 @@ def create_solver(device: str, global_bounding_box, global_num_cell, density: fl
                   body: Optional[BodyForce], delta_t: float,
                   matrix_solver: str = "Reference", preconditioner: str = "Jacobi",
                   **options) -> Solver:
+    """createSolver (src/Solver.hpp:283-350): a Solver for solver_config's
+    configuration of the same arguments."""
+    return Solver(solver_config(device, global_bounding_box, global_num_cell, density, bc,
+                                source, body, delta_t, matrix_solver, preconditioner,
+                                **options))
+
+
+def solver_config(device: str, global_bounding_box, global_num_cell, density: float,
+                  bc: BoundaryCondition, source: Optional[InflowSource],
+                  body: Optional[BodyForce], delta_t: float,
+                  matrix_solver: str = "Reference", preconditioner: str = "Jacobi",
+                  **options) -> dict:
     """createSolver (src/Solver.hpp:283-350).  device must be "hip" (the only
-    backend); matrix_solver "Reference" is the Jacobi-PCG.  options: any
-    cf_config field (field_order, cg_tol, precision, quirks, periodic, ...)."""
+    backend).  Solver choice as createVelocityCorrector
+    (src/VelocityCorrector.hpp:297-336): matrix_solver "Reference" is the
+    Jacobi-PCG (preconditioner ignored, as there); "PCG" takes the
+    preconditioner "PFMG" (the reference example's default,
+    examples/advection.cpp:186-189), served by this build's geometric
+    multigrid V-cycle, or "Jacobi".  Other HYPRE solvers and preconditioners
+    are not provided.  options: any cf_config field (field_order, cg_tol,
+    precision, quirks, periodic, mg_nu, ...)."""
     if device not in ("hip", "gfx950"):
         raise RuntimeError("invalid backend: only the HIP (gfx950) backend exists")
-    if matrix_solver != "Reference":
-        raise RuntimeError(f"matrix solver {matrix_solver!r} is not provided "
-                           "(only the Reference Jacobi-PCG)")
+    if matrix_solver == "Reference":
+        precond = PRECOND_JACOBI
+    elif matrix_solver == "PCG" and preconditioner in ("PFMG", "Jacobi"):
+        precond = PRECOND_MG if preconditioner == "PFMG" else PRECOND_JACOBI
+    else:
+        raise RuntimeError(f"matrix solver {matrix_solver!r} with preconditioner "
+                           f"{preconditioner!r} is not provided (Reference, or PCG with "
+                           "PFMG or Jacobi)")
     dim = len(global_num_cell)
     cfg = base_config(dim, global_num_cell[0])
     cfg["num_cell"] = list(global_num_cell) + [1] * (3 - dim)
@@ -298,6 +321,7 @@ def create_solver(device: str, global_bounding_box, global_num_cell, density: fl
     cfg["wall_type"] = [[bt[d], bt[d + dim]] if d < dim else [SOLID, SOLID] for d in range(3)]
     cfg["density"] = density
     cfg["dt"] = delta_t
+    cfg["preconditioner"] = precond
     if source is not None:
         cfg["inflow_shape"] = INFLOW_BOX
         cfg["inflow_location"] = list(source.location) + [0.0] * (3 - dim)
@@ -312,4 +336,4 @@ def create_solver(device: str, global_bounding_box, global_num_cell, density: fl
         if k not in cfg:
             raise KeyError(f"unknown option {k!r}")
         cfg[k] = v
-    return Solver(cfg)
+    return cfg

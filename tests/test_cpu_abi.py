@@ -77,7 +77,7 @@ def test_invalid_configs_rejected_before_any_launch(bad):
 
 @pytest.mark.parametrize("bad", [
     dict(preconditioner=2), dict(mg_nu=0), dict(mg_coarse_sweeps=0),
-    dict(mg_omega=0.0), dict(mg_omega=1.5),
+    dict(mg_omega=0.0), dict(mg_omega=1.5), dict(mg_alpha=0.0), dict(mg_alpha=2.5),
 ])
 def test_invalid_multigrid_configs(bad):
     cfg = K.base_config(2, 8)
@@ -110,3 +110,20 @@ def test_create_without_device_fails_cleanly():
     assert rc == _lib.CF_EDEVICE
     assert not h.value
     assert L.cf_last_error(None)
+
+
+def test_create_solver_choices():
+    """createVelocityCorrector's choice (src/VelocityCorrector.hpp:297-336):
+    "Reference" is the Jacobi-PCG whatever the preconditioner; "PCG" takes
+    "PFMG" (the example's default) or "Jacobi"; others are refused."""
+    from cajitafluids_amd.solver import BoundaryCondition, solver_config
+    args = ("hip", [0.0, 0.0, 1.0, 1.0], [16, 16], 0.1, BoundaryCondition(), None, None, 0.005)
+    assert solver_config(*args, "Reference", "PFMG")["preconditioner"] == K.PRECOND_JACOBI
+    assert solver_config(*args, "PCG", "PFMG")["preconditioner"] == K.PRECOND_MG
+    assert solver_config(*args, "PCG", "Jacobi")["preconditioner"] == K.PRECOND_JACOBI
+    for bad in (("GMRES", "PFMG"), ("PCG", "SMG"), ("PCG", "None")):
+        with pytest.raises(RuntimeError):
+            solver_config(*args, *bad)
+    cfg = solver_config(*args, "PCG", "PFMG", mg_nu=3)
+    assert cfg["mg_nu"] == 3
+    layout(cfg, K.CELL)  # the configuration is valid
