@@ -148,6 +148,10 @@ SIGNATURES = {
     "cf_kernel_time": (C.c_int, [_P, C.c_int, _dp, C.POINTER(C.c_int64)]),
     "cf_reset_kernel_time": (C.c_int, [_P]),
     "cf_adjoint_run": (C.c_int, [_P, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "cf_cell_output": (C.c_int, [_P, _dp, _dp]),
+    "cf_write_output": (C.c_int, [_P, C.c_char_p, C.c_int]),
+    "cf_write_vti": (C.c_int, [C.c_char_p, C.c_int, _ip, _dp, C.c_double, C.c_int,
+                               C.c_double, C.c_double, _dp, _dp]),
 }
 
 _lib = None

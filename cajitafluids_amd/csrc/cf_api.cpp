@@ -442,4 +442,46 @@ cf_status cf_adjoint_run(cf_solver* h, int nsteps, int ckpt,
     });
 }
 
+cf_status cf_cell_output(cf_solver* h, double* q_host, double* vel_host)
+{
+    return guarded(h, [&](cfmi::Solver& s) {
+        if (!q_host || !vel_host)
+            throw CfError(CF_EINVAL, "q_host and vel_host must be non-null");
+        s.cellOutput(q_host, vel_host);
+        return CF_OK;
+    });
+}
+
+cf_status cf_write_output(cf_solver* h, const char* dir, int step)
+{
+    return guarded(h, [&](cfmi::Solver& s) {
+        if (!dir || step < 0)
+            throw CfError(CF_EINVAL, "dir must be non-null and step >= 0");
+        cfmi::write_output(s, dir, step);
+        return CF_OK;
+    });
+}
+
+// No handle: errors are reported through cf_last_error(NULL).
+cf_status cf_write_vti(const char* path, int dim, const int n[3], const double origin[3],
+                       double dx, int cycle, double time, double dt, const double* q,
+                       const double* vel)
+{
+    try {
+        if (!path || (dim != 2 && dim != 3) || !n || !origin || !q || !vel)
+            throw CfError(CF_EINVAL, "cf_write_vti: bad arguments");
+        for (int d = 0; d < dim; ++d)
+            if (n[d] <= 0)
+                throw CfError(CF_EINVAL, "cf_write_vti: n must be positive");
+        cfmi::write_vti(path, dim, n, origin, dx, cycle, time, dt, q, vel);
+        return CF_OK;
+    } catch (const CfError& e) {
+        g_create_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        return CF_EINVAL;
+    }
+}
+
 } // extern "C"

@@ -246,4 +246,40 @@ void launch_project(const Geo& g, Fields cur, const double* p, hipStream_t s)
         hipLaunchKernelGGL(k_project<3>, union_grid(g, blk), blk, 0, s, g, cur, p);
 }
 
+// SiloWriter.hpp:141-197: the owned cells of q and, at each owned cell
+// centre, the velocity from linear interpolation of each face component
+// (interpolateVelocity<2, 1>), as compact arrays with x fastest; vel_out
+// holds dim arrays back to back.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_cell_output(Geo g, Fields cur, double* __restrict__ q_out,
+                                                     double* __restrict__ vel_out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.n[0] || j >= g.n[1])
+        return;
+    const int l[3] = {g.h + i, g.h + j, g.own_lo(2) + k};
+    double x[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+        x[d] = coord(g, CF_CELL, d, l[d]);
+    const int64_t o = i + (int64_t)g.n[0] * (j + (int64_t)g.n[1] * k);
+    const int64_t N = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    q_out[o] = cur.e[CF_CELL][g.idx(l[0], l[1], l[2])];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+        vel_out[d * N + o] = interpolate<DIM, 1>(g, d + 1, cur.e[d + 1], x);
+}
+
+void launch_cell_output(const Geo& g, Fields cur, double* q_out, double* vel_out, hipStream_t s)
+{
+    dim3 blk(64, 4, 1);
+    dim3 grid((g.n[0] + 63) / 64, (g.n[1] + 3) / 4, g.n[2]);
+    if (g.dim == 2)
+        hipLaunchKernelGGL(k_cell_output<2>, grid, blk, 0, s, g, cur, q_out, vel_out);
+    else
+        hipLaunchKernelGGL(k_cell_output<3>, grid, blk, 0, s, g, cur, q_out, vel_out);
+}
+
 } // namespace cfmi

@@ -51,6 +51,8 @@ void launch_rhs_sum(const Geo& g, const double* rhs, double* out_sum,
 void launch_rhs_shift(const Geo& g, double* rhs, const double* sum,
                       hipStream_t s);
 void launch_project(const Geo& g, Fields cur, const double* p, hipStream_t s);
+// The output writer's cell data (owned q, cell-centre velocity; compact).
+void launch_cell_output(const Geo& g, Fields cur, double* q_out, double* vel_out, hipStream_t s);
 
 // Jacobi-PCG (cf_pcg.hip).  init64: r = b, x = 0; init32: d = 0 for the
 // FP32 inner solve of the residual already in r.

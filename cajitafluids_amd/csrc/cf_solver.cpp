@@ -613,4 +613,17 @@ void Solver::initRandomFaces(uint64_t seed)
     launch_init_random(_mesh->geo(), seed, _pm->fields(CF_CURRENT), _s);
 }
 
+void Solver::cellOutput(double* q_host, double* vel_host)
+{
+    const Geo& g = _mesh->geo();
+    const int64_t N = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    DevArray<double> buf;
+    buf.alloc((1 + g.dim) * N, _s);
+    launch_cell_output(g, _pm->fields(CF_CURRENT), buf.ptr, buf.ptr + N, _s);
+    CF_HIP(hipMemcpyAsync(q_host, buf.ptr, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, _s));
+    CF_HIP(hipMemcpyAsync(vel_host, buf.ptr + N, (size_t)(g.dim * N) * sizeof(double),
+                          hipMemcpyDeviceToHost, _s));
+    CF_HIP(hipStreamSynchronize(_s));
+}
+
 } // namespace cfmi

@@ -222,6 +222,9 @@ class Solver
     cf_status adjointRun(int nsteps, int ckpt, const double* dq_T,
                          double* const* grad, double dvel[3]);
 
+    // The output writer's cell data into host arrays (cf_output.cpp).
+    void cellOutput(double* q_host, double* vel_host);
+
     void initVortexTubes(uint64_t seed, double peak);
     void initBlobs(uint64_t seed, int nblobs);
     void initRandomFaces(uint64_t seed);
@@ -236,6 +239,11 @@ class Solver
     DevArray<unsigned long long> _vmax;
     double _time = 0.0;
 };
+
+// The output writer (cf_output.cpp).
+void write_vti(const std::string& path, int dim, const int n[3], const double origin[3], double dx,
+               int cycle, double time, double dt, const double* q, const double* vel);
+void write_output(Solver& s, const std::string& dir, int step);
 
 // TimeIntegrator::step (src/TimeIntegrator.hpp:120-177): gather, advect all
 // entities into Next in one launch, swap.

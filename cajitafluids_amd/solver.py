@@ -101,10 +101,33 @@ class Solver:
     def run(self, nsteps):
         return self._stats_call(self.L.cf_run, nsteps, what="run")
 
-    def solve(self, nsteps):
-        """Solver::solve without output: setup, then exactly nsteps steps."""
+    def solve_steps(self, nsteps):
+        """setup, then exactly nsteps steps (no output)."""
         self.setup()
         return self.run(nsteps)
+
+    def solve(self, t_final, write_freq, output_dir="data", log=print):
+        """Solver::solve (src/Solver.hpp:148-176): write step 0, setup, then
+        step until time >= t_final; every write_freq-th loop counter t logs
+        a line and, after that step, writes output labelled t (the state
+        after t + 1 steps, as there).  output_dir None (or write_freq <= 0)
+        writes nothing.  Returns the number of steps taken."""
+        write = output_dir is not None and write_freq > 0
+        if write:
+            self.write_output(output_dir, 0)
+        num_step = int(t_final / self.dt)
+        self.setup()
+        t = 0
+        while True:
+            due = write_freq > 0 and t % write_freq == 0
+            if due and log is not None:
+                log("Step %d / %d at time = %f" % (t, num_step, self.time))
+            self.step()
+            if due and write:
+                self.write_output(output_dir, t)
+            t += 1
+            if not self.time < t_final:
+                return t
 
     @property
     def dt(self):
@@ -232,6 +255,30 @@ class Solver:
     def reset_kernel_time(self):
         self._call(self.L.cf_reset_kernel_time, what="reset_kernel_time")
 
+    # ---- output (SiloWriter) -------------------------------------------------------
+    def cell_output(self):
+        """SiloWriter::writeFile's data: (q, [vel_x, vel_y(, vel_z)]), the
+        owned q and the cell-centre velocity from linear interpolation of the
+        faces, each an (n2, n1, n0) array."""
+        g = self._owned_cells()
+        N = g[0] * g[1] * g[2]
+        q = np.zeros(N)
+        vel = np.zeros(self.dim * N)
+        self._call(self.L.cf_cell_output, q.ctypes.data_as(_lib._dp),
+                   vel.ctypes.data_as(_lib._dp), what="cell_output")
+        shape = (g[2], g[1], g[0])
+        return q.reshape(shape), [vel[d * N:(d + 1) * N].reshape(shape) for d in range(self.dim)]
+
+    def write_output(self, directory, step):
+        """SiloWriter::siloWrite: <directory>/CajitaFluids<step>.pvti naming
+        <directory>/raw/CajitaFluidsOutput00000<step>.vti (VTK XML)."""
+        self._call(self.L.cf_write_output, str(directory).encode(), int(step),
+                   what="write_output")
+
+    def _owned_cells(self):
+        lo, hi = self.owned(CELL)
+        return [hi[d] - lo[d] for d in range(3)]
+
     # ---- adjoint ---------------------------------------------------------------
     def adjoint_run(self, nsteps, ckpt, dq_T):
         """Reverse mode of setup + nsteps steps; returns dL/d(initial state)
@@ -322,6 +369,9 @@ def solver_config(device: str, global_bounding_box, global_num_cell, density: fl
     cfg["density"] = density
     cfg["dt"] = delta_t
     cfg["preconditioner"] = precond
+    # the pressure solver's settings (VelocityCorrector.hpp:103-104)
+    cfg["cg_tol"] = 1.0e-6
+    cfg["cg_max_iter"] = 2000
     if source is not None:
         cfg["inflow_shape"] = INFLOW_BOX
         cfg["inflow_location"] = list(source.location) + [0.0] * (3 - dim)

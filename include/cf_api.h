@@ -214,6 +214,24 @@ cf_status cf_adjoint_run(cf_solver* h, int nsteps, int ckpt,
                          double* dv0_host, double* dw0_host, double* dq0_host,
                          double dvel[3]);
 
+/* ---- output (replaces SiloWriter, src/SiloWriter.hpp:32-422) ------------ */
+/* SiloWriter::writeFile's data (SiloWriter.hpp:141-197): the owned cells of
+ * q (Current) and the cell-centre velocity from linear interpolation of each
+ * face component, as compact host arrays with x fastest: q_host n0*n1*n2,
+ * vel_host dim arrays of n0*n1*n2 back to back. */
+cf_status cf_cell_output(cf_solver* h, double* q_host, double* vel_host);
+/* SiloWriter::siloWrite (SiloWriter.hpp:352-418) in VTK XML instead of Silo:
+ * <dir>/raw/CajitaFluidsOutput00000<step:05d>.vti (ImageData: CellData
+ * "quantity" and "velocity" (3 components), FieldData CYCLE / TIME / DTIME)
+ * and the master <dir>/CajitaFluids<step:05d>.pvti naming it; creates dir
+ * and dir/raw. */
+cf_status cf_write_output(cf_solver* h, const char* dir, int step);
+/* The piece writer alone, from host arrays laid out as cf_cell_output's
+ * (no handle, no device use); origin: the domain's low corner. */
+cf_status cf_write_vti(const char* path, int dim, const int n[3], const double origin[3],
+                       double dx, int cycle, double time, double dt, const double* q,
+                       const double* vel);
+
 #ifdef __cplusplus
 }
 #endif

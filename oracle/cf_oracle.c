@@ -1264,3 +1264,29 @@ void cfo_init_random_faces(cfo_state* s, uint64_t seed)
                 }
     }
 }
+
+/* SiloWriter::writeFile's data (SiloWriter.hpp:141-197): the owned cells of
+ * q (Current) and, at each owned cell centre, the velocity from linear
+ * (order 1) interpolation of each face component (interpolateVelocity<2, 1>,
+ * Interpolation.hpp:45-54), as compact arrays with x fastest (the writer's
+ * LayoutLeft order).  vel_out holds dim arrays back to back. */
+void cfo_cell_output(const cfo_state* s, double* q_out, double* vel_out)
+{
+    int lo[3], hi[3];
+    cfo_owned_range(s, CFO_CELL, lo, hi);
+    const int n0 = hi[0] - lo[0], n1 = hi[1] - lo[1], n2 = hi[2] - lo[2];
+    const long N = (long)n0 * n1 * n2;
+    const double* q = s->f[CFO_CELL][0];
+    for (int k = lo[2]; k < hi[2]; ++k)
+        for (int j = lo[1]; j < hi[1]; ++j)
+            for (int i = lo[0]; i < hi[0]; ++i) {
+                long o = (long)(i - lo[0]) + (long)n0 * ((j - lo[1]) + (long)n1 * (k - lo[2]));
+                int l[3] = {i, j, k};
+                double x[3] = {0.0, 0.0, 0.0};
+                for (int d = 0; d < s->dim; ++d)
+                    x[d] = cfo_coord(s, CFO_CELL, d, l[d]);
+                q_out[o] = q[cfo_idx(s, CFO_CELL, i, j, k)];
+                for (int d = 0; d < s->dim; ++d)
+                    vel_out[d * N + o] = cfo_interpolate(s, d + 1, 1, s->f[d + 1][0], x);
+            }
+}

@@ -140,6 +140,10 @@ void cfo_rk3(const cfo_state* s, const double x0[3], const double* u,
 void cfo_apply_A(cfo_state* s, const double* x, double* y); /* y = A x (owned) */
 double cfo_dot(const cfo_state* s, const double* a, const double* b);
 
+/* The writer's cell data: owned q and the linearly interpolated cell-centre
+ * velocity, compact with x fastest (vel_out: dim arrays back to back). */
+void cfo_cell_output(const cfo_state* s, double* q_out, double* vel_out);
+
 /* Seeded generators (counter-based SplitMix64). */
 uint64_t cfo_rand_u64(uint64_t seed, uint64_t stream, uint64_t idx);
 double cfo_rand_uniform(uint64_t seed, uint64_t stream, uint64_t idx);

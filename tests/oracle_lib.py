@@ -134,6 +134,7 @@ def lib():
         "cfo_advance": (None, [P, C.c_int]),
         "cfo_spline_weights": (None, [C.c_int, C.c_double, ip, dp]),
         "cfo_interpolate": (C.c_double, [P, C.c_int, C.c_int, dp, dp]),
+        "cfo_cell_output": (None, [P, dp, dp]),
         "cfo_rk3": (None, [P, dp, dp, dp, dp, C.c_double, dp]),
         "cfo_apply_A": (None, [P, dp, dp]),
         "cfo_dot": (C.c_double, [P, dp, dp]),
@@ -282,6 +283,18 @@ class Oracle:
 
     def dot(self, a, b):
         return self.L.cfo_dot(self.h, _dp(a), _dp(b))
+
+    def cell_output(self):
+        """(q, vel): owned q and the cell-centre velocity, each as
+        (n2, n1, n0) arrays (vel: dim of them)."""
+        lo, hi = self.owned(CELL)
+        n = [hi[d] - lo[d] for d in range(3)]
+        N = n[0] * n[1] * n[2]
+        q = np.zeros(N)
+        vel = np.zeros(self.dim * N)
+        self.L.cfo_cell_output(self.h, _dp(q), _dp(vel))
+        shape = (n[2], n[1], n[0])
+        return q.reshape(shape), [vel[d * N:(d + 1) * N].reshape(shape) for d in range(self.dim)]
 
     def interpolate(self, entity, order, field, p):
         p = np.ascontiguousarray(np.asarray(list(p) + [0.0] * (3 - len(p)), dtype=np.float64))
