@@ -42,6 +42,14 @@ def parse():
                     help="skip the multigrid-preconditioned FP64 leg")
     ap.add_argument("--mg-only", action="store_true",
                     help="only the multigrid leg (profiling runs; prints its object)")
+    ap.add_argument("--mg-nu", type=int, default=None,
+                    help="multigrid leg: sweeps before / after the coarse correction")
+    ap.add_argument("--mg-alpha", type=float, default=None,
+                    help="multigrid leg: coarse-correction scale")
+    ap.add_argument("--mg-omega", type=float, default=None,
+                    help="multigrid leg: Jacobi damping")
+    ap.add_argument("--mg-coarse", type=int, default=None,
+                    help="multigrid leg: sweeps on the coarsest level")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=400)
     ap.add_argument("--timing-sample", type=int, default=8,
@@ -300,6 +308,10 @@ def mg_leg(n, args, value64, q64, p64):
     from cajitafluids_amd import _lib, configs as K
     cfg = K.vortex_tank_3d(n)
     cfg["preconditioner"] = K.PRECOND_MG
+    for key, val in (("mg_nu", args.mg_nu), ("mg_alpha", args.mg_alpha),
+                     ("mg_omega", args.mg_omega), ("mg_coarse_sweeps", args.mg_coarse)):
+        if val is not None:
+            cfg[key] = val
     sol = make_solver(cfg)
     t, its, _, res, kt = timed_run(sol, args.steps, args.warmup, timing=args.timing_sample)
     qm, pm = owned_q_and_p(sol) if q64 is not None else (None, None)
@@ -319,7 +331,7 @@ def mg_leg(n, args, value64, q64, p64):
         "iterations_per_step": round(sum(its) / args.steps, 1),
         "max_final_residual": max(res),
         "mg_nu": cfg["mg_nu"], "mg_omega": cfg["mg_omega"],
-        "mg_coarse_sweeps": cfg["mg_coarse_sweeps"],
+        "mg_coarse_sweeps": cfg["mg_coarse_sweeps"], "mg_alpha": cfg["mg_alpha"],
         "dye_rel_l2_vs_jacobi": rel(qm, q64) if qm is not None else None,
         "pressure_rel_l2_vs_jacobi": rel(pm, p64) if pm is not None else None,
         "kernels": kern,

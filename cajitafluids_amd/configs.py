@@ -48,10 +48,10 @@ def base_config(dim=2, n=128):
         quirks=QUIRK_RK3_V0 | (QUIRK_PROJ_BC_U if dim == 2 else 0),
         refine_mode=REFINE_CARRY,
         preconditioner=PRECOND_JACOBI,
-        mg_nu=2,
-        mg_omega=0.8,
+        mg_nu=3,
+        mg_omega=0.9,
         mg_coarse_sweeps=16,
-        mg_alpha=1.8,
+        mg_alpha=1.9,
     )
 
 

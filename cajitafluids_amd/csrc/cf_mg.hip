@@ -55,11 +55,13 @@ __global__ __launch_bounds__(256) void k_mg0_first(Geo g, MgTab wd, const double
     });
 }
 
-// One Jacobi sweep xo = fma(w, b - A x, x), x from SRC; RZ adds r.z.
+// One Jacobi sweep xo = fma(w, b - A x, x) on a padded level (row kernel,
+// one pass over memory), x from SRC (the parent of a cell at pm); RZ adds
+// r.z.
 template <int DIM, int SRC, bool RZ>
 __global__ __launch_bounds__(256) void k_mg0_smooth(Geo g, MgTab wd, const double* __restrict__ xi,
-                                                    const double* __restrict__ b, MgLev c1,
-                                                    const double* __restrict__ xc,
+                                                    const double* __restrict__ b, MgParentMap pm,
+                                                    double alpha, const double* __restrict__ xc,
                                                     double* __restrict__ xo, PcgState* st,
                                                     RedBuf rb)
 {
@@ -67,7 +69,7 @@ __global__ __launch_bounds__(256) void k_mg0_smooth(Geo g, MgTab wd, const doubl
         return;
     double hi[1] = {0.0}, lo[1] = {0.0};
     const double s = offdiag<double>(g);
-    // level 0 -> 1 coarsens every axis with more than 2 cells
+    // the next level coarsens every axis with more than 2 cells
     const bool cx = g.n[0] > 2, cy = g.n[1] > 2, cz = DIM > 2 && g.n[2] > 2;
     for_owned_rows(g, [&](int64_t base, int j, int k) {
         const RowStencil rs = row_stencil(g, j, k);
@@ -76,9 +78,9 @@ __global__ __launch_bounds__(256) void k_mg0_smooth(Geo g, MgTab wd, const doubl
                            wrap_nbr(g.periodic[1], g.n[1], j, +1)};
         const int kn[2] = {DIM > 2 ? wrap_nbr(g.periodic[2], g.n[2], k, -1) : 0,
                            DIM > 2 ? wrap_nbr(g.periodic[2], g.n[2], k, +1) : 0};
-        auto parent = [&](int i0, int j0, int k0) -> int {
+        auto parent = [&](int i0, int j0, int k0) -> int64_t {
             const int P0 = cx ? i0 >> 1 : i0, P1 = cy ? j0 >> 1 : j0, P2 = cz ? k0 >> 1 : k0;
-            return P0 + c1.n[0] * (P1 + c1.n[1] * P2);
+            return pm.base + P0 + pm.sy * P1 + pm.sz * P2;
         };
         // value at a stencil point (0 at a ghost across a wall)
         auto val = [&](int64_t at, bool wall, int cnt, int i0, int j0, int k0) -> double {
@@ -87,7 +89,7 @@ __global__ __launch_bounds__(256) void k_mg0_smooth(Geo g, MgTab wd, const doubl
             } else if constexpr (SRC == MG_FIRST) {
                 return wall ? 0.0 : tab_sel(wd, cnt) * b[at];
             } else {
-                return wall ? 0.0 : fma(c1.alpha, xc[parent(i0, j0, k0)], xi[at]);
+                return wall ? 0.0 : fma(alpha, xc[parent(i0, j0, k0)], xi[at]);
             }
         };
         for (int i = threadIdx.x; i < g.n[0]; i += kBlock) {
@@ -324,8 +326,11 @@ __device__ __forceinline__ int flat(const int (&n)[3], int i, int j, int k)
 #define CF_GRID_STRIDE(id, count) \
     for (int id = blockIdx.x * blockDim.x + threadIdx.x; id < (count); id += gridDim.x * blockDim.x)
 
+// b_c[parent] = sum over children (z, y, x order) of (b - A x) on a padded
+// level; one thread per parent of C (written at pm).
 template <int DIM>
-__global__ __launch_bounds__(256) void k_mg0_resrestrict(Geo g, MgLev C, const double* __restrict__ x,
+__global__ __launch_bounds__(256) void k_mg0_resrestrict(Geo g, MgLev C, MgParentMap pm,
+                                                         const double* __restrict__ x,
                                                          const double* __restrict__ b,
                                                          double* __restrict__ bc,
                                                          const PcgState* st)
@@ -349,7 +354,7 @@ __global__ __launch_bounds__(256) void k_mg0_resrestrict(Geo g, MgLev C, const d
                     sum = sum + (b[c] - apply_A<DIM, double>(g, rs, lx, x, c));
                 }
             }
-        bc[id] = sum;
+        bc[pm.base + P[0] + pm.sy * P[1] + pm.sz * P[2]] = sum;
     }
 }
 
@@ -740,12 +745,13 @@ void launch_mg0_first(const Geo& g, const MgTab& wd0, const double* b, double* x
 
 template <int DIM>
 static void mg0_smooth_dim(const Geo& g, const MgTab& wd0, int src, bool rz, const double* xi,
-                           const double* b, const MgLev& c1, const double* xc, double* xo,
-                           PcgState* st, RedBuf rb, hipStream_t s)
+                           const double* b, MgParentMap pm, double alpha, const double* xc,
+                           double* xo, PcgState* st, RedBuf rb, hipStream_t s)
 {
     const dim3 grid(stream_blocks(g)), blk(kBlock);
-#define CF_MG0_SMOOTH(SRC, RZ) \
-    hipLaunchKernelGGL((k_mg0_smooth<DIM, SRC, RZ>), grid, blk, 0, s, g, wd0, xi, b, c1, xc, xo, st, rb)
+#define CF_MG0_SMOOTH(SRC, RZ)                                                                   \
+    hipLaunchKernelGGL((k_mg0_smooth<DIM, SRC, RZ>), grid, blk, 0, s, g, wd0, xi, b, pm, alpha, \
+                       xc, xo, st, rb)
     if (src == MG_FIRST) {
         if (rz)
             CF_MG0_SMOOTH(MG_FIRST, true);
@@ -766,13 +772,13 @@ static void mg0_smooth_dim(const Geo& g, const MgTab& wd0, int src, bool rz, con
 }
 
 void launch_mg0_smooth(const Geo& g, const MgTab& wd0, int src, bool rz, const double* xi,
-                       const double* b, const MgLev& c1, const double* xc, double* xo,
-                       PcgState* st, RedBuf rb, hipStream_t s)
+                       const double* b, MgParentMap pm, double alpha, const double* xc,
+                       double* xo, PcgState* st, RedBuf rb, hipStream_t s)
 {
     if (g.dim == 2)
-        mg0_smooth_dim<2>(g, wd0, src, rz, xi, b, c1, xc, xo, st, rb, s);
+        mg0_smooth_dim<2>(g, wd0, src, rz, xi, b, pm, alpha, xc, xo, st, rb, s);
     else
-        mg0_smooth_dim<3>(g, wd0, src, rz, xi, b, c1, xc, xo, st, rb, s);
+        mg0_smooth_dim<3>(g, wd0, src, rz, xi, b, pm, alpha, xc, xo, st, rb, s);
 }
 
 // the tile grid of the tiled multigrid kernels: an even z chunk
@@ -830,15 +836,15 @@ void launch_mg_resrestrict3(const Geo& g, const double* x, const double* b, doub
                        pm, x, b, bc, st);
 }
 
-void launch_mg0_resrestrict(const Geo& g, const MgLev& c1, const double* x, const double* b,
-                            double* b1, const PcgState* st, hipStream_t s)
+void launch_mg0_resrestrict(const Geo& g, const MgLev& c1, MgParentMap pm, const double* x,
+                            const double* b, double* b1, const PcgState* st, hipStream_t s)
 {
     if (g.dim == 2)
         hipLaunchKernelGGL(k_mg0_resrestrict<2>, compact_grid(c1.ncell), dim3(kBlock), 0, s, g,
-                           c1, x, b, b1, st);
+                           c1, pm, x, b, b1, st);
     else
         hipLaunchKernelGGL(k_mg0_resrestrict<3>, compact_grid(c1.ncell), dim3(kBlock), 0, s, g,
-                           c1, x, b, b1, st);
+                           c1, pm, x, b, b1, st);
 }
 
 void launch_mg_first(const MgLev& L, const double* b, double* x, const PcgState* st,

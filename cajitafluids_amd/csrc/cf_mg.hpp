@@ -67,6 +67,13 @@ struct MgTail
     double* res[kMgMaxLevels];
 };
 
+// Linear index of the parent of cell (i, j, k) in the next level's array:
+// base + i/2 + sy j/2 + sz k/2 (compact or padded layout).
+struct MgParentMap
+{
+    int64_t base, sy, sz;
+};
+
 // Source of the values a smoothing sweep reads:
 //   first:   x = w b (the sweep from zero, computed on the fly)
 //   plain:   x = xi
@@ -78,25 +85,20 @@ enum MgSrc
     MG_PROLONG = 2
 };
 
-// Level 0 (padded layout): xo = fma(w, b - A x, x) with x from `src`; with
-// rz, also r.z = b.xo reduced into st (beta = r.z / rho after the first
-// iteration, rho = r.z).  launch_mg0_first: x = w b alone (a single sweep
-// from zero).
+// Padded levels by row kernels (level 0 of 2-D or thin grids, and small
+// tiled levels): xo = fma(w, b - A x, x) with x from `src` (the parent's
+// value read at pm, scaled by alpha); with rz, also r.z = b.xo reduced into
+// st (beta = r.z / rho after the first iteration, rho = r.z).
+// launch_mg0_first: x = w b alone (a single sweep from zero).
 void launch_mg0_smooth(const Geo& g, const MgTab& wd0, int src, bool rz, const double* xi,
-                       const double* b, const MgLev& c1, const double* xc, double* xo,
-                       PcgState* st, RedBuf rb, hipStream_t s);
+                       const double* b, MgParentMap pm, double alpha, const double* xc,
+                       double* xo, PcgState* st, RedBuf rb, hipStream_t s);
 void launch_mg0_first(const Geo& g, const MgTab& wd0, const double* b, double* x,
                       const PcgState* st, hipStream_t s);
-// Level 0 -> 1: b1 = sum over children (z, y, x order) of (b - A x).
-void launch_mg0_resrestrict(const Geo& g, const MgLev& c1, const double* x, const double* b,
-                            double* b1, const PcgState* st, hipStream_t s);
-
-// Linear index of the parent of cell (i, j, k) in the next level's array:
-// base + i/2 + sy j/2 + sz k/2 (compact or padded layout).
-struct MgParentMap
-{
-    int64_t base, sy, sz;
-};
+// Next level's b1 = sum over children (z, y, x order) of (b - A x); c1: the
+// next level's dims, pm: where its cells are stored.
+void launch_mg0_resrestrict(const Geo& g, const MgLev& c1, MgParentMap pm, const double* x,
+                            const double* b, double* b1, const PcgState* st, hipStream_t s);
 
 // Tiled levels (3-D, every axis coarsened further): level 0 and the coarse
 // levels of full 2^l boxes in a padded layout with their own Geo.  The same
@@ -146,6 +148,7 @@ class Multigrid
     void smooth(int l, int src, bool rz, const double* xi, const double* b, double* xo,
                 PcgState* st, RedBuf rb);
     void resrestrict(int l, const double* x, const double* b, PcgState* st);
+    bool marched(int l) const;
     const Geo& _g;
     hipStream_t _s;
     int _nu, _coarse;

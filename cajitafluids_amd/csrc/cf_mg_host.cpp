@@ -233,17 +233,30 @@ void Multigrid::first(int l, const double* b, double* x, PcgState* st)
         launch_mg_first(_lev[l], b, x, st, _s);
 }
 
+// Sweeps on padded levels of at most this many cells run the row kernels:
+// one pass over memory per launch, where the z march's prologue and
+// plane-by-plane chain make a small grid latency-bound (measured at 32^3:
+// 4.5 against 7.5 µs; at 64^3 the march is faster).  Restrictions keep the
+// march, whose lane-pair sums beat the row kernel's thread per parent.
+constexpr int64_t kMgMarchMinCells = 1 << 15;
+
+bool Multigrid::marched(int l) const
+{
+    return l < _nt && (l == 0 || (int64_t)_lev[l].ncell > kMgMarchMinCells);
+}
+
 // one sweep on level l (l below the coarsest); the prolongation source is
 // level l + 1's cycle result
 void Multigrid::smooth(int l, int src, bool rz, const double* xi, const double* b, double* xo,
                        PcgState* st, RedBuf rb)
 {
     const double* xc = _buf->res[l + 1];
-    if (l < _nt)
-        launch_mg_smooth3(_lg[l], _lgw[l], src, rz, xi, b, xc, _pm[l], _lev[0].alpha, xo, st, rb,
-                          _s);
-    else if (l == 0)
-        launch_mg0_smooth(_g, _wt[0], src, rz, xi, b, _lev[1], xc, xo, st, rb, _s);
+    const double alpha = _lev[0].alpha;
+    if (marched(l))
+        launch_mg_smooth3(_lg[l], _lgw[l], src, rz, xi, b, xc, _pm[l], alpha, xo, st, rb, _s);
+    else if (l == 0 || l < _nt)
+        launch_mg0_smooth(l == 0 ? _g : _lg[l], _wt[l], src, rz, xi, b, _pm[l], alpha, xc, xo,
+                          st, rb, _s);
     else
         launch_mg_smooth(_lev[l], src, xi, b, _lev[l + 1], xc, xo, st, _s);
 }
@@ -255,7 +268,7 @@ void Multigrid::resrestrict(int l, const double* x, const double* b, PcgState* s
     if (l < _nt)
         launch_mg_resrestrict3(_lg[l], x, b, bc, _pm[l], st, _s);
     else if (l == 0)
-        launch_mg0_resrestrict(_g, _lev[1], x, b, bc, st, _s);
+        launch_mg0_resrestrict(_g, _lev[1], _pm[0], x, b, bc, st, _s);
     else
         launch_mg_resrestrict(_lev[l], _lev[l + 1], x, b, bc, st, _s);
 }
@@ -268,13 +281,14 @@ const double* Multigrid::vcycle(const double* r, PcgState* st, RedBuf rb)
         // level 0 is the coarsest: coarse sweeps from zero
         double* x = B.X[0];
         double* t = B.T[0];
+        const MgParentMap none{0, 0, 0};
         if (_coarse == 1)
             first(0, r, x, st);
         else
-            launch_mg0_smooth(_g, _wt[0], MG_FIRST, false, nullptr, r, _lev[0], nullptr, x, st,
+            launch_mg0_smooth(_g, _wt[0], MG_FIRST, false, nullptr, r, none, 0.0, nullptr, x, st,
                               rb, _s);
         for (int it = 2; it < _coarse; ++it) {
-            launch_mg0_smooth(_g, _wt[0], MG_PLAIN, false, x, r, _lev[0], nullptr, t, st, rb,
+            launch_mg0_smooth(_g, _wt[0], MG_PLAIN, false, x, r, none, 0.0, nullptr, t, st, rb,
                               _s);
             std::swap(x, t);
         }

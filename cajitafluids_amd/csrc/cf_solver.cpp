@@ -466,9 +466,10 @@ cf_status VelocityCorrector::solveMg(const double* b, double* x, cf_solve_stats*
     readState(hs);
     int cur = 0, launched = 0;
     int batch = _predict_mg > 3 ? _predict_mg - 2 : 8;
+    const int it0 = hs.iter;
     while (!hs.done) {
         for (int c = 0; c < batch; ++c) {
-            const int it = hs.iter + launched + c + 1;
+            const int it = it0 + launched + c + 1;
             int t0 = _timer.begin_iter(_s, it);
             launch_pcg_dir_spmv<double>(_g_unit, z, _p[cur].ptr, _p[cur ^ 1].ptr, _st, rb, _s);
             _timer.end(t0, _s, TK_DIR_SPMV, it);

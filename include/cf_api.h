@@ -93,7 +93,7 @@ typedef struct cf_config {
     double mg_omega;      /* Jacobi damping in (0, 1] */
     int mg_coarse_sweeps; /* Jacobi sweeps on the coarsest level (>= 1) */
     double mg_alpha;      /* scale of the coarse-grid correction, x += alpha
-                             P x_c, in (0, 2]: over-correction (about 1.8)
+                             P x_c, in (0, 2]: over-correction (about 1.9)
                              offsets the piecewise-constant prolongation */
 } cf_config;
 

@@ -58,6 +58,9 @@ CASES = {
     "3d_periodic_xz_48x32x40": _cfg(3, (48, 32, 40), periodic=(1, 0, 1)),
     "3d_free_top_64": _cfg(3, (64, 64, 64),
                            walls=[[SOLID, SOLID], [SOLID, SOLID], [SOLID, FREE]]),
+    # a coarse level large enough for the z-marching kernels (66^3), then
+    # padded row-kernel levels (33^3) and compact ones (17^3)
+    "3d_periodic_x_132": _cfg(3, (132, 132, 132), periodic=(1, 0, 0)),
 }
 
 
@@ -93,7 +96,7 @@ def test_mg_pcg_matches_oracle(cf, oracle_mod, name):
 @pytest.mark.parametrize("name", ["2d_free_45x27", "3d_ragged_70x36x40",
                                   "3d_periodic_xy_21x18x11", "3d_nu1_coarse4_20",
                                   "3d_alpha1_nu3_24", "3d_odd_45",
-                                  "3d_periodic_xz_48x32x40"])
+                                  "3d_periodic_xz_48x32x40", "3d_periodic_x_132"])
 def test_mg_pcg_truncated(cf, oracle_mod, name, maxit):
     from cajitafluids_amd import _lib
     cfg = K.copy_config(CASES[name])
