@@ -252,22 +252,25 @@ cf_status cf_pressure_view(cf_solver* h, int which, cf_view* out)
 }
 
 // Dense ghosted host array <-> padded device layout, one 2-D copy per plane.
+// One contiguous host transfer through a dense device buffer, repacked on
+// the device (plane-by-plane 2-D copies from pageable memory ran at about
+// 3 ms per plane).
 static void copy_dense(cfmi::Solver& s, double* dev, int entity, void* host,
                        bool to_device)
 {
     const cfmi::Geo& g = s.geo();
     const int E0 = g.ext[entity][0], E1 = g.ext[entity][1], E2 = g.ext[entity][2];
-    for (int k = 0; k < E2; ++k) {
-        double* d = dev + g.sx + g.px * g.py * (int64_t)k;
-        char* hp = (char*)host + sizeof(double) * (size_t)E0 * E1 * k;
-        if (to_device)
-            CF_HIP(hipMemcpy2DAsync(d, g.px * sizeof(double), hp,
-                                    E0 * sizeof(double), E0 * sizeof(double), E1,
-                                    hipMemcpyHostToDevice, s.stream()));
-        else
-            CF_HIP(hipMemcpy2DAsync(hp, E0 * sizeof(double), d,
-                                    g.px * sizeof(double), E0 * sizeof(double),
-                                    E1, hipMemcpyDeviceToHost, s.stream()));
+    const size_t n = (size_t)E0 * E1 * E2;
+    cfmi::DevArray<double> dense;
+    dense.alloc((int64_t)n, s.stream());
+    if (to_device) {
+        CF_HIP(hipMemcpyAsync(dense.ptr, host, n * sizeof(double), hipMemcpyHostToDevice,
+                              s.stream()));
+        cfmi::launch_repack(g, entity, dev, dense.ptr, true, s.stream());
+    } else {
+        cfmi::launch_repack(g, entity, dev, dense.ptr, false, s.stream());
+        CF_HIP(hipMemcpyAsync(host, dense.ptr, n * sizeof(double), hipMemcpyDeviceToHost,
+                              s.stream()));
     }
     CF_HIP(hipStreamSynchronize(s.stream()));
 }

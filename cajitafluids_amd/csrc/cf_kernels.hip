@@ -289,6 +289,35 @@ __global__ __launch_bounds__(256) void k_cell_output(Geo g, Fields cur, double* 
         vel_out[d * N + o] = interpolate<DIM, 1>(g, d + 1, cur.e[d + 1], x);
 }
 
+// Dense ghosted array (x fastest, the entity's ghosted extents) <-> the
+// padded layout: the host transfers of cf_copy_in / cf_copy_out are one
+// contiguous copy each, with this repacking on the device.
+__global__ __launch_bounds__(256) void k_repack(Geo g, int e, double* __restrict__ padded,
+                                                double* __restrict__ dense, int to_padded)
+{
+    const int E0 = g.ext[e][0], E1 = g.ext[e][1], E2 = g.ext[e][2];
+    const int64_t total = (int64_t)E0 * E1 * E2;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t % E0);
+        const int64_t r = t / E0;
+        const int j = (int)(r % E1);
+        const int k = (int)(r / E1);
+        const int64_t c = g.idx(i, j, k);
+        if (to_padded)
+            padded[c] = dense[t];
+        else
+            dense[t] = padded[c];
+    }
+}
+
+void launch_repack(const Geo& g, int entity, double* padded, double* dense, bool to_padded,
+                   hipStream_t s)
+{
+    hipLaunchKernelGGL(k_repack, dim3(2048), dim3(256), 0, s, g, entity, padded, dense,
+                       to_padded ? 1 : 0);
+}
+
 void launch_cell_output(const Geo& g, Fields cur, double* q_out, double* vel_out, hipStream_t s)
 {
     dim3 blk(64, 4, 1);

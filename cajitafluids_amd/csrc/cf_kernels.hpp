@@ -51,6 +51,9 @@ void launch_rhs_sum(const Geo& g, const double* rhs, double* out_sum,
 void launch_rhs_shift(const Geo& g, double* rhs, const double* sum,
                       hipStream_t s);
 void launch_project(const Geo& g, Fields cur, const double* p, hipStream_t s);
+// Dense ghosted array <-> padded layout (host transfers).
+void launch_repack(const Geo& g, int entity, double* padded, double* dense, bool to_padded,
+                   hipStream_t s);
 // The output writer's cell data (owned q, cell-centre velocity; compact).
 void launch_cell_output(const Geo& g, Fields cur, double* q_out, double* vel_out, hipStream_t s);
 
