@@ -164,14 +164,28 @@ def test_mg_full_steps(cf, oracle_mod, case):
     orc.close()
 
 
-def test_mg_adjoint_matches_oracle(cf, oracle_mod):
+def _adjoint_case(name):
+    if name == "2d_tank":
+        cfg = K.adjoint_tank_2d(24)
+        cfg.update(cg_tol=1e-12, inflow_velocity=[0.5, 3.0, 0.0])
+        return cfg, 4, 2
+    cfg = K.periodic_box_3d(12)
+    cfg.update(cg_tol=1e-12, precision=K.FP64, inflow_shape=K.INFLOW_SPHERE,
+               inflow_center=[0.5, 0.5, 0.3], inflow_radius=0.25,
+               inflow_velocity=[0.3, -0.2, 2.0], inflow_quantity=1.0, clamp_dt=0, dt=0.01)
+    return cfg, 3, 2
+
+
+@pytest.mark.parametrize("name", ["2d_tank", "3d_periodic_box_12"])
+def test_mg_adjoint_matches_oracle(cf, oracle_mod, name):
     """The reverse mode's pressure solves go through the same preconditioner
     on both sides."""
-    cfg = K.adjoint_tank_2d(24)
-    cfg.update(cg_tol=1e-12, inflow_velocity=[0.5, 3.0, 0.0], preconditioner=K.PRECOND_MG)
+    cfg, nsteps, ckpt = _adjoint_case(name)
+    cfg["preconditioner"] = K.PRECOND_MG
+    dim = cfg["dim"]
     rng = np.random.default_rng(7)
     init = {}
-    for e in range(3):
+    for e in range(dim + 1):
         ext, lo, hi = cf.layout(cfg, e)
         a = np.zeros((ext[2], ext[1], ext[0]))
         sl = (slice(lo[2], hi[2]), slice(lo[1], hi[1]), slice(lo[0], hi[0]))
@@ -181,13 +195,13 @@ def test_mg_adjoint_matches_oracle(cf, oracle_mod):
     with cf.Solver(cfg) as sol:
         for e, a in init.items():
             sol.set(e, a)
-        g = sol.adjoint_run(4, 2, w)
+        g = sol.adjoint_run(nsteps, ckpt, w)
     orc = oracle_mod.Oracle(cfg)
     for e, a in init.items():
         orc.field(e)[...] = a
-    rc, go = orc.adjoint_run(4, 2, w)
+    rc, go = orc.adjoint_run(nsteps, ckpt, w)
     assert rc == 0
-    for e, k in enumerate(("dq0", "du", "dv")):
+    for e, k in enumerate(("dq0", "du", "dv", "dw")[:dim + 1]):
         _, lo, hi = cf.layout(cfg, e)
         sl = (slice(lo[2], hi[2]), slice(lo[1], hi[1]), slice(lo[0], hi[0]))
         assert rel_l2(g[k][sl], go[k][sl]) <= 1e-10, k
