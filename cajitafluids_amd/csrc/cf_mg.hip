@@ -150,38 +150,35 @@ struct MgSweepEpi
     const double* __restrict__ b;
     double* __restrict__ xo;
     double hi[1] = {0.0}, lo[1] = {0.0};
-    double bc[tile3::RY], bn[tile3::RY];
-    template <class PL>
+    double bb[2][tile3::RY]; // b of a plane, alternating with the march's steps
+    template <int B, class PL>
     __device__ void take_raw(const PL& P)
     {
 #pragma unroll
         for (int q = 0; q < tile3::RY; ++q)
-            bn[q] = P.a[q][0];
+            bb[B][q] = P.a[q][0];
     }
+    template <int B>
     __device__ void prefetch(const Tile3Ctx<1>& c, const Plane3Meta& m, int k)
     {
         if constexpr (!RAW) {
 #pragma unroll
             for (int q = 0; q < tile3::RY; ++q) {
                 const bool ok = k < c.k1 && c.valid[q][0];
-                bn[q] = ok ? b[c.colb[q] + m.off] : 0.0;
+                bb[B][q] = load_any<double>(b, ok, c.colb[q] + m.off, c.safe); // raw: read on valid rows only
             }
         }
     }
-    __device__ void rotate()
-    {
-#pragma unroll
-        for (int q = 0; q < tile3::RY; ++q)
-            bc[q] = bn[q];
-    }
+    __device__ void rotate() {}
+    template <int B>
     __device__ void cells(const Tile3Ctx<1>& c, const Plane3Meta& m, int q, const double (&pc)[1],
                           const double (&y)[1])
     {
         const double w = dinv_pick<double>(gw, c.wy[q] + m.wz, c.wx[0]);
-        const double out = fma(w, bc[q] - y[0], pc[0]);
+        const double out = fma(w, bb[B][q] - y[0], pc[0]);
         xo[c.colb[q] + m.off] = out;
         if constexpr (RZ)
-            dd_acc_prod<false>(hi[0], lo[0], bc[q], out);
+            dd_acc_prod<false>(hi[0], lo[0], bb[B][q], out);
     }
 };
 
@@ -225,9 +222,11 @@ struct MgRestrictEpi
     double* __restrict__ bc;
     MgParentMap pm;
     const Tile3Ctx<1>* ctx = nullptr;
-    double bcur[tile3::RY], bn[tile3::RY], r[tile3::RY] = {0.0, 0.0, 0.0, 0.0};
+    // b of a plane, alternating with the march's steps
+    double bb[2][tile3::RY], r[tile3::RY] = {0.0, 0.0, 0.0, 0.0};
     double sum[tile3::RY / 2] = {0.0, 0.0};
     int knext = -1, kcur = -1;
+    template <int B>
     __device__ void prefetch(const Tile3Ctx<1>& c, const Plane3Meta& m, int k)
     {
         ctx = &c;
@@ -235,13 +234,14 @@ struct MgRestrictEpi
 #pragma unroll
         for (int q = 0; q < tile3::RY; ++q) {
             const bool ok = k < c.k1 && c.valid[q][0];
-            bn[q] = ok ? b[c.colb[q] + m.off] : 0.0;
+            bb[B][q] = load_any<double>(b, ok, c.colb[q] + m.off, c.safe); // raw: read on valid rows only
         }
     }
+    template <int B>
     __device__ void cells(const Tile3Ctx<1>&, const Plane3Meta&, int q, const double (&)[1],
                           const double (&y)[1])
     {
-        r[q] = bcur[q] - y[0];
+        r[q] = bb[B][q] - y[0];
     }
     __device__ void rotate()
     {
@@ -278,9 +278,6 @@ struct MgRestrictEpi
             }
         }
         kcur = knext;
-#pragma unroll
-        for (int q = 0; q < tile3::RY; ++q)
-            bcur[q] = bn[q];
     }
 };
 

@@ -219,8 +219,13 @@ struct DirEpiT
     const Geo& g;
     T* __restrict__ p_new;
     double hi[1] = {0.0}, lo[1] = {0.0};
-    __device__ void prefetch(const Tile3Ctx<CPL>&, const Plane3Meta&, int) {}
+    template <int B>
+    __device__ void prefetch(const Tile3Ctx<CPL>&, const Plane3Meta&, int)
+    {
+    }
+    __device__ void shift() {}
     __device__ void rotate() {}
+    template <int B>
     __device__ void cells(const Tile3Ctx<CPL>& c, const Plane3Meta& m, int q,
                           const T (&pc)[CPL], const T (&y)[CPL])
     {
@@ -240,7 +245,9 @@ struct UpdEpiT
     T* __restrict__ r;
     T alpha;
     double hi[2] = {0.0, 0.0}, lo[2] = {0.0, 0.0};
-    T xc[tile3::RY][CPL], rc[tile3::RY][CPL], xn[tile3::RY][CPL], rn_[tile3::RY][CPL];
+    // x and r of a plane, in two buffers that alternate with the march's steps
+    T xb[2][tile3::RY][CPL], rb[2][tile3::RY][CPL];
+    template <int B>
     __device__ void prefetch(const Tile3Ctx<CPL>& c, const Plane3Meta& m, int k)
     {
 #pragma unroll
@@ -248,27 +255,35 @@ struct UpdEpiT
             // a group with a valid first cell lies inside its padded row
             const bool ok = k < c.k1 && c.valid[q][0];
             const int64_t at = c.colb[q] + m.off;
+            if constexpr (CPL == 1) {
+                // raw: cells() reads only valid rows of planes below k1
+                xb[B][q][0] = load_any<T>(x, ok, at, c.safe);
+                rb[B][q][0] = load_any<T>(r, ok, at, c.safe);
+                continue;
+            }
 #pragma unroll
             for (int j = 0; j < CPL; ++j) {
-                xn[q][j] = (T)0;
-                rn_[q][j] = (T)0;
+                xb[B][q][j] = (T)0;
+                rb[B][q][j] = (T)0;
             }
             if (ok) {
-                vload<T, CPL>(x + at, xn[q]);
-                vload<T, CPL>(r + at, rn_[q]);
+                vload<T, CPL>(x + at, xb[B][q]);
+                vload<T, CPL>(r + at, rb[B][q]);
             }
         }
     }
-    __device__ void rotate()
+    __device__ void shift()
     {
 #pragma unroll
         for (int q = 0; q < tile3::RY; ++q)
 #pragma unroll
             for (int j = 0; j < CPL; ++j) {
-                xc[q][j] = xn[q][j];
-                rc[q][j] = rn_[q][j];
+                xb[0][q][j] = xb[1][q][j];
+                rb[0][q][j] = rb[1][q][j];
             }
     }
+    __device__ void rotate() {}
+    template <int B>
     __device__ void cells(const Tile3Ctx<CPL>& c, const Plane3Meta& m, int q,
                           const T (&pc)[CPL], const T (&qv)[CPL])
     {
@@ -276,8 +291,8 @@ struct UpdEpiT
         T xo[CPL], ro[CPL];
 #pragma unroll
         for (int j = 0; j < CPL; ++j) {
-            xo[j] = fma_t<T>(alpha, pc[j], xc[q][j]);
-            ro[j] = fma_t<T>(-alpha, qv[j], rc[q][j]);
+            xo[j] = fma_t<T>(alpha, pc[j], xb[B][q][j]);
+            ro[j] = fma_t<T>(-alpha, qv[j], rb[B][q][j]);
         }
         vstore_valid<T, CPL>(x + at, xo, c.valid[q]);
         vstore_valid<T, CPL>(r + at, ro, c.valid[q]);

@@ -14,6 +14,16 @@
 // halo cells come from the neighbouring tiles, which the XCD-ordered block
 // mapping keeps in the same L2.
 //
+// Loads that feed the pipeline are issued on every path (a lane with
+// nothing to read loads a cell known to be in bounds and discards it): the
+// compiler counts vector memory operations per path, and one conditional
+// load makes it wait for every load in flight, the next plane's included,
+// before touching the current one.  The loaded registers are kept raw and
+// masked where they are used: a select right after the load may be placed
+// at the end of the loop body, where it waits for the loads just as well.
+// FP32 groups of two cells (CPL = 2) still branch around their wrap and
+// x-halo cases.
+//
 // Everything that depends on the lane's x position is computed once per
 // block, everything that depends on a row once per block and wave (rows are
 // wave-uniform), and everything that depends on the plane once per plane, so
@@ -96,6 +106,15 @@ inline int tile3_blocks(const Geo& g)
     return (t.work + 7) / 8 * 8;
 }
 
+// v[at] where ok, else v[safe] (an index known to be in bounds), for a
+// caller that masks or ignores the !ok lanes' value where it uses it.
+template <class T>
+__device__ __forceinline__ T load_any(const T* __restrict__ v, bool ok, int64_t at,
+                                      int64_t safe)
+{
+    return v[ok ? at : safe];
+}
+
 // Maps a global index that may lie one cell outside the domain to the
 // stored cell it stands for: wrapped on a periodic axis; -1 (a zero ghost)
 // otherwise, and for anything further out.
@@ -109,15 +128,15 @@ __device__ __forceinline__ int wrap_or_ghost(const Geo& g, int d, int gi)
     return -1;
 }
 
-// Jacobi diagonal inverse for a per-lane wall count (at most 6 walls).
+// Jacobi diagonal inverse for a wave-uniform z wall count wz and a per-lane
+// x + y wall count wxy in [0, 4]: five uniform table reads and four selects
+// (a table indexed by the per-lane count would be a vector memory read).
 template <class T>
-__device__ __forceinline__ T dinv_sel(const Geo& g, int cnt)
+__device__ __forceinline__ T dinv_sel(const Geo& g, int wz, int wxy)
 {
-    T v = dinv_of<T>(g, 0);
-#pragma unroll
-    for (int q = 1; q <= 6; ++q)
-        v = cnt == q ? dinv_of<T>(g, q) : v;
-    return v;
+    const T a = dinv_of<T>(g, wz), b = dinv_of<T>(g, wz + 1), c = dinv_of<T>(g, wz + 2),
+            d = dinv_of<T>(g, wz + 3), e = dinv_of<T>(g, wz + 4);
+    return wxy == 0 ? a : wxy == 1 ? b : wxy == 2 ? c : wxy == 3 ? d : e;
 }
 
 // Whether the -/+ coupling of an owned cell with global index gi along axis
@@ -209,6 +228,7 @@ struct Tile3Ctx
     int i0, j0, k0, k1;
     int lane, w;
     int64_t colb[tile3::NS];
+    int64_t safe;                       // an owned cell: the address of discarded loads
     int sgi[tile3::NS], sgj[tile3::NS]; // (wrapped) x, y of each slot's first cell
     // a group whose last cell is the row's end, on a periodic x axis: its
     // second cell (one past the end) is the row's first cell (CPL = 2)
@@ -242,6 +262,7 @@ __device__ __forceinline__ Tile3Ctx<CPL> tile3_ctx(const Geo& g, const Tile3Grid
     c.k1 = min(c.k0 + t.kz, g.n[2]);
     c.lane = threadIdx.x & (LX - 1);
     c.w = __builtin_amdgcn_readfirstlane(threadIdx.x / LX);
+    c.safe = g.idx(g.h, g.h, g.h);
     auto col = [&](int gi, int gj) -> int64_t {
         const int a = wrap_or_ghost(g, 0, gi), b = wrap_or_ghost(g, 1, gj);
         if (a < 0 || b < 0)
@@ -377,12 +398,17 @@ __device__ __forceinline__ void tile3_load(const Tile3Ctx<CPL>& c, const Plane3M
         const bool single = CPL > 1 && s == tile3::RY && c.hkind == 2;
         const bool split = CPL > 1 && (s < tile3::RY ? c.split : c.hsplit);
         if constexpr (MODE == 2) {
-            P.b[s][0] = (T)0;
-            if (ok) {
-                P.a[s][0] = va[at];
-                P.b[s][0] = par.xc[par.base + (c.sgi[s] >> 1) + par.sy * (c.sgj[s] >> 1) +
-                                   par.sz * (m.zc >> 1)];
-            }
+            const int64_t pat = par.base + (c.sgi[s] >> 1) + par.sy * (c.sgj[s] >> 1) +
+                                par.sz * (m.zc >> 1);
+            P.a[s][0] = load_any<T>(va, ok, at, c.safe);
+            P.b[s][0] = load_any<T>(par.xc, ok, pat, par.base);
+            continue;
+        } else if constexpr (CPL == 1) {
+            // the direction's old value is read on the first iteration too
+            // (and ignored there): a load on every path
+            P.a[s][0] = load_any<T>(va, ok, at, c.safe);
+            if constexpr (DIR)
+                P.b[s][0] = load_any<T>(vb, ok, at, c.safe);
             continue;
         }
         if (ok && !single && !split) {
@@ -416,19 +442,24 @@ __device__ __forceinline__ T tile3_value(const Geo& g, const Tile3Ctx<CPL>& c,
                                          const Plane3<T, MODE, CPL>& P, int s, int j,
                                          bool first, T beta)
 {
+    T v;
     if constexpr (MODE == 0) {
-        return P.a[s][j];
+        v = P.a[s][j];
     } else if constexpr (MODE == 2) {
-        return fma_t<T>(beta, P.b[s][j], P.a[s][j]);
+        v = fma_t<T>(beta, P.b[s][j], P.a[s][j]);
     } else {
         T dinv;
         if (s < tile3::RY)
             dinv = dinv_pick<T>(g, c.wy[s] + m.wz, c.wx[j]); // uniform y, z part
         else
-            dinv = dinv_sel<T>(g, c.hwxy[j] + m.wz);
+            dinv = dinv_sel<T>(g, m.wz, c.hwxy[j]);
         const T z = dinv * P.a[s][j];
-        return first ? z : fma_t<T>(beta, P.b[s][j], z);
+        v = first ? z : fma_t<T>(beta, P.b[s][j], z);
     }
+    // CPL = 1 loads are raw: a zero ghost slot read some owned cell
+    if constexpr (CPL == 1)
+        return c.colb[s] >= 0 && m.off >= 0 ? v : (T)0;
+    return v;
 }
 
 // Writes the plane's values into LDS plane pl and own[]; the halo slot
@@ -490,8 +521,21 @@ __device__ __forceinline__ T tile3_apply(const Geo& g, const Tile3Ctx<CPL>& c,
 }
 
 // The z march shared by the direction (MODE 1), update (MODE 0) and
-// multigrid kernels; epi.cells() receives each own row's centre values and
+// multigrid kernels; epi.cells<B>() receives each own row's centre values and
 // A applied to the values.
+//
+// With one cell per lane the loop is unrolled by two with ping-pong
+// buffers, so no register copy of a plane's loads (which would wait for
+// them) happens before their use: at step s (plane k = k0 + s) the march's
+// plane k+1 is in P[s & 1], plane k+2 loads into P[(s + 1) & 1], and the
+// epilogue's data of plane k sits in its buffer s & 1 while plane k+1's
+// loads into (s + 1) & 1.  With two cells per lane (FP32) the second set of
+// live buffers costs more registers than the overlap gains: that loop keeps
+// one buffer pair and copies (P0 = P1, epi.shift()) after each plane.
+// Epilogue interface: prefetch<B>(c, m, k) loads plane k's data into buffer
+// B (k >= k1: values never used), take_raw<B>(P) takes the marched operand
+// instead, cells<B>() uses buffer B, shift() copies buffer 1 into 0 (CPL = 2
+// only), rotate() runs after every step's barrier.
 template <class T, int DIR, int CPL, class Epi>
 __device__ __forceinline__ void tile3_march(
     const Geo& g, const Tile3Grid& tg, const T* __restrict__ va,
@@ -500,10 +544,15 @@ __device__ __forceinline__ void tile3_march(
     const Tile3Parent par = Tile3Parent{nullptr, 0, 0, 0, 0.0})
 {
     using namespace tile3;
+    using E = std::decay_t<Epi>;
     const Tile3Ctx<CPL> c = tile3_ctx<CPL>(g, tg);
     if (!c.active)
         return;
     const bool need_b = DIR == 2 || !first;
+    // MODE 1 reads vb on every path (first iteration included, where its
+    // values are ignored); a caller without an old direction gets va's
+    if (vb == nullptr)
+        vb = va;
     Plane3<T, DIR, CPL> P0, P1;
     T pm[RY][CPL], pc[RY][CPL], pp[RY][CPL];
     // prologue: plane k0-1 (own column only), plane k0 into LDS, k0+1 in flight
@@ -519,21 +568,23 @@ __device__ __forceinline__ void tile3_march(
     tile3_finish<T, DIR, CPL>(g, c, mc, P1, first, beta, pl[0], pc);
     Plane3Meta mn = plane3_meta(g, c.k0 + 1);
     tile3_load<T, DIR, CPL>(c, mn, va, vb, need_b, par, P0);
-    if constexpr (epi_takes_raw<std::decay_t<Epi>>::value)
-        epi.take_raw(P1);
-    epi.prefetch(c, mc, c.k0);
+    if constexpr (epi_takes_raw<E>::value)
+        epi.template take_raw<0>(P1);
+    epi.template prefetch<0>(c, mc, c.k0);
     epi.rotate();
     __syncthreads();
-    int cur = 0;
-    for (int k = c.k0; k < c.k1; ++k) {
-        // P0 holds plane k+1 (loaded last step); start plane k+2
-        const Plane3Meta mnn = plane3_meta(g, k + 2);
-        if (k + 2 <= c.k1)
-            tile3_load<T, DIR, CPL>(c, mnn, va, vb, need_b, par, P1);
-        tile3_finish<T, DIR, CPL>(g, c, mn, P0, first, beta, pl[cur ^ 1], pp);
-        if constexpr (epi_takes_raw<std::decay_t<Epi>>::value)
-            epi.take_raw(P0);
-        epi.prefetch(c, mn, k + 1);
+    auto step = [&](int k, auto parity) {
+        constexpr int S = decltype(parity)::value;
+        Plane3<T, DIR, CPL>& Pf = S == 0 ? P0 : P1; // plane k+1
+        Plane3<T, DIR, CPL>& Pl = S == 0 ? P1 : P0; // plane k+2, started here
+        // the last step loads plane k1 again (an L2 hit, never used) rather
+        // than branching around the loads
+        const Plane3Meta mnn = plane3_meta(g, min(k + 2, c.k1));
+        tile3_load<T, DIR, CPL>(c, mnn, va, vb, need_b, par, Pl);
+        tile3_finish<T, DIR, CPL>(g, c, mn, Pf, first, beta, pl[S ^ 1], pp);
+        if constexpr (epi_takes_raw<E>::value)
+            epi.template take_raw<S ^ 1>(Pf);
+        epi.template prefetch<S ^ 1>(c, mn, k + 1);
 #pragma unroll
         for (int q = 0; q < RY; ++q) {
             // a group with no valid cell has nothing to compute or store
@@ -541,9 +592,9 @@ __device__ __forceinline__ void tile3_march(
                 T y[CPL];
 #pragma unroll
                 for (int j = 0; j < CPL; ++j)
-                    y[j] = tile3_apply<T, CPL>(g, c, mc, pl[cur], q, j, pc[q], pm[q][j],
+                    y[j] = tile3_apply<T, CPL>(g, c, mc, pl[S], q, j, pc[q], pm[q][j],
                                                pp[q][j]);
-                epi.cells(c, mc, q, pc[q], y);
+                epi.template cells<S>(c, mc, q, pc[q], y);
             }
 #pragma unroll
             for (int j = 0; j < CPL; ++j) {
@@ -552,11 +603,50 @@ __device__ __forceinline__ void tile3_march(
             }
         }
         __syncthreads();
-        cur ^= 1;
-        P0 = P1;
         mc = mn;
         mn = mnn;
         epi.rotate();
+    };
+    if constexpr (CPL == 1) {
+        int k = c.k0;
+        for (; k + 1 < c.k1; k += 2) {
+            step(k, std::integral_constant<int, 0>{});
+            step(k + 1, std::integral_constant<int, 1>{});
+        }
+        if (k < c.k1)
+            step(k, std::integral_constant<int, 0>{});
+    } else {
+        int cur = 0;
+        for (int k = c.k0; k < c.k1; ++k) {
+            const Plane3Meta mnn = plane3_meta(g, k + 2);
+            if (k + 2 <= c.k1)
+                tile3_load<T, DIR, CPL>(c, mnn, va, vb, need_b, par, P1);
+            tile3_finish<T, DIR, CPL>(g, c, mn, P0, first, beta, pl[cur ^ 1], pp);
+            epi.template prefetch<1>(c, mn, k + 1);
+#pragma unroll
+            for (int q = 0; q < RY; ++q) {
+                if (c.valid[q][0]) {
+                    T y[CPL];
+#pragma unroll
+                    for (int j = 0; j < CPL; ++j)
+                        y[j] = tile3_apply<T, CPL>(g, c, mc, pl[cur], q, j, pc[q], pm[q][j],
+                                                   pp[q][j]);
+                    epi.template cells<0>(c, mc, q, pc[q], y);
+                }
+#pragma unroll
+                for (int j = 0; j < CPL; ++j) {
+                    pm[q][j] = pc[q][j];
+                    pc[q][j] = pp[q][j];
+                }
+            }
+            __syncthreads();
+            cur ^= 1;
+            P0 = P1;
+            mc = mn;
+            mn = mnn;
+            epi.shift();
+            epi.rotate();
+        }
     }
 }
 
