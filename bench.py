@@ -67,6 +67,10 @@ def parse():
     ap.add_argument("--summarize-trace", metavar="CSV",
                     help="print per-kernel, per-grid-size dispatch count and mean / total "
                          "duration of a rocprofv3 --kernel-trace CSV, then exit (no GPU use)")
+    ap.add_argument("--stream-probe", action="store_true",
+                    help="only time plain streaming kernels (torch's add: read 2, write "
+                         "1; and copy) over 263^3 FP64 arrays, the padded 256^3 size, "
+                         "and print their rates")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a plain 1 GiB device copy, the reference "
                          "kernel for scaling rocprof FETCH_SIZE / WRITE_SIZE")
@@ -83,6 +87,36 @@ def copy_probe():
     b.copy_(a)
     torch.cuda.synchronize()
     del a, b
+
+
+def stream_probe():
+    """Rates of plain streaming kernels over arrays of one padded 256^3 FP64
+    vector (263^3 elements): torch's add (c = a + s b: read 2, write 1) and
+    copy (read 1, write 1), each the mean of 50 launches that alternate
+    between two operand sets so that one launch's inputs are not the
+    previous launch's outputs.  The reference rate for the PCG kernels'
+    streams."""
+    import json
+    import torch
+    n = 263 ** 3
+    a, b, c, d, e = (torch.full((n,), 1.0 + i, dtype=torch.float64, device="cuda")
+                     for i in range(5))
+    out = {}
+    for name, words, fn in (
+            ("add_2r1w", 3, lambda i: torch.add(a, b, alpha=0.5, out=c) if i & 1
+             else torch.add(d, e, alpha=0.5, out=b)),
+            ("copy_1r1w", 2, lambda i: c.copy_(a) if i & 1 else b.copy_(d))):
+        fn(0)
+        torch.cuda.synchronize()
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record()
+        for i in range(50):
+            fn(i)
+        t1.record()
+        t1.synchronize()
+        us = t0.elapsed_time(t1) * 1e3 / 50
+        out[name] = {"us": round(us, 1), "tb_s": round(words * n * 8 / us / 1e6, 2)}
+    print(json.dumps({"stream_probe": out, "elements": n}), flush=True)
 
 
 def make_solver(cfg):
@@ -148,7 +182,7 @@ def summarize_trace(path):
 
 
 def summarize_counters(path):
-    """Per-kernel dispatch count, median and sum of each counter in a
+    """Per-kernel dispatch count, median, max and sum of each counter in a
     rocprofv3 counter_collection CSV, largest sum first."""
     import csv
     import statistics
@@ -158,9 +192,9 @@ def summarize_counters(path):
             vals.setdefault((r["Kernel_Name"], r["Counter_Name"]), []).append(
                 float(r["Counter_Value"]))
     w = csv.writer(sys.stdout)
-    w.writerow(["kernel", "counter", "dispatches", "median", "sum"])
+    w.writerow(["kernel", "counter", "dispatches", "median", "max", "sum"])
     for (k, c), v in sorted(vals.items(), key=lambda kv: -sum(kv[1])):
-        w.writerow([k[:120], c, len(v), statistics.median(v), sum(v)])
+        w.writerow([k[:120], c, len(v), statistics.median(v), max(v), sum(v)])
 
 
 def main():
@@ -172,14 +206,18 @@ def main():
         summarize_trace(args.summarize_trace)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; the "
-                          "multi-GPU design is described in DESIGN.md",
+        print(json.dumps({"error": "this library runs one subdomain on one GPU; "
+                          "several GPUs (spatial blocks with halo exchange) "
+                          "are not supported",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     from cajitafluids_amd import _lib, configs as K
 
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_legs()}))
+        return
+    if args.stream_probe:
+        stream_probe()
         return
     if args.copy_probe:
         copy_probe()

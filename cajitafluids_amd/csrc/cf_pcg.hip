@@ -2,7 +2,7 @@
 // defect-correction loop) for the pressure solve of
 // src/VelocityCorrector.hpp:266-282.
 //
-// Control flow (this build's statement of Cajita's ReferenceConjugateGradient,
+// Control flow (this library's statement of Cajita's ReferenceConjugateGradient,
 // called at src/VelocityCorrector.hpp:276), from x = 0:
 //   r = b; rho = r.(D^-1 r); bnorm = ||b||            (bnorm = 0: done)
 //   iteration k:

@@ -343,7 +343,7 @@ def solver_config(device: str, global_bounding_box, global_num_cell, density: fl
     (src/VelocityCorrector.hpp:297-336): matrix_solver "Reference" is the
     Jacobi-PCG (preconditioner ignored, as there); "PCG" takes the
     preconditioner "PFMG" (the reference example's default,
-    examples/advection.cpp:186-189), served by this build's geometric
+    examples/advection.cpp:186-189), served by the library's geometric
     multigrid V-cycle, or "Jacobi".  Other HYPRE solvers and preconditioners
     are not provided.  options: any cf_config field (field_order, cg_tol,
     precision, quirks, periodic, mg_nu, ...)."""

@@ -28,7 +28,7 @@ typedef int cf_status;
 #define CF_ENOMEM 2   /* device or host allocation failed */
 #define CF_ENOCONV 3  /* pressure solve hit its iteration limit */
 #define CF_EDEVICE 4  /* HIP runtime error, or no usable gfx950 device */
-#define CF_ENOTIMPL 5 /* combination not provided by this build */
+#define CF_ENOTIMPL 5 /* combination not provided by this library */
 
 /* ---- tags (Cajita entity / ProblemManager version tags) ---------------- */
 enum { CF_CELL = 0, CF_FACE_X = 1, CF_FACE_Y = 2, CF_FACE_Z = 3 };

@@ -673,7 +673,7 @@ static double cfo_dot32(const struct cfo_state* s, const float* a,
 
 
 /*
- * Jacobi-PCG, as this build defines Cajita's ReferenceConjugateGradient:
+ * Jacobi-PCG, as the oracle defines Cajita's ReferenceConjugateGradient:
  *   r = b - A x; z = M r; p = z; rho = r.z; bnorm = ||b||
  *   (bnorm == 0: x = 0, converged in 0 iterations)
  *   for k = 1..maxit:
