@@ -399,6 +399,8 @@ def adjoint_leg(cfg, nsteps, ckpt, init, loss_grad):
         g = s.adjoint_run(nsteps, ckpt, dqT)
         t_adj = time.perf_counter() - t0
     cells = int(np.prod([c for c in cfg["num_cell"][:cfg["dim"]]]))
+    print(f"adjoint leg: {cells} cells, {nsteps} steps: forward {t_fwd:.3f} s, "
+          f"adjoint {t_adj:.3f} s", file=sys.stderr, flush=True)
     return {"cells": cells, "steps": nsteps, "ckpt": ckpt,
             "forward_s": round(t_fwd, 3), "adjoint_s": round(t_adj, 3),
             "adjoint_over_forward": round(t_adj / t_fwd, 2),
