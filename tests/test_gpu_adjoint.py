@@ -103,12 +103,29 @@ def _case(name):
         c.update(cg_tol=1e-10, clamp_dt=0, dt=0.01, inflow_radius=0.3,
                  inflow_velocity=[0.3, 0.2, 2.0])
         return c, 3, 1
+    # Traces of several cells (CFL 3.2 |u|, 6.4 in the inflow): the 3-D
+    # advection adjoint's samples leave its block's LDS window (3 cells
+    # around a 32 x 4 x 2 tile) and take the direct global scatter; on the
+    # periodic box the window's flush also wraps.
+    if name == "3d_tank_high_cfl":
+        c = K.vortex_tank_3d(16)
+        c.update(cg_tol=1e-12, clamp_dt=0, dt=0.2, inflow_radius=0.3,
+                 inflow_velocity=[0.3, 0.2, 2.0])
+        return c, 3, 2
+    if name == "3d_periodic_high_cfl":
+        c = K.periodic_box_3d(12)
+        c.update(cg_tol=1e-12, precision=K.FP64, inflow_shape=K.INFLOW_SPHERE,
+                 inflow_center=[0.5, 0.5, 0.3], inflow_radius=0.25,
+                 inflow_velocity=[0.3, -0.2, 2.0], inflow_quantity=1.0,
+                 clamp_dt=0, dt=0.2)
+        return c, 3, 2
     raise KeyError(name)
 
 
 CASES = ["2d_linear_quirks", "2d_cubic_cubic", "2d_no_quirks", "2d_periodic_x",
          "2d_periodic_xy_cubic", "2d_free_wall", "3d_periodic_xy_cubic",
-         "3d_tank_quirks", "3d_tank_fp32_refined"]
+         "3d_tank_quirks", "3d_tank_fp32_refined", "3d_tank_high_cfl",
+         "3d_periodic_high_cfl"]
 
 
 @pytest.mark.parametrize("name", CASES)
