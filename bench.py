@@ -1,4 +1,4 @@
-"""Benchmark: cell-steps/s of the per-timestep hot path at 256^3 (config 2),
+"""Benchmark: cell-steps/s of the per-timestep hot path on the 3-D 256^3 vortex tank,
 the PCG's achieved HBM bandwidth, and the FP32-refined solver against the
 FP64 one at the same 1e-8 relative residual.
 
@@ -123,7 +123,7 @@ def make_solver(cfg):
     from cajitafluids_amd import configs as K
     from cajitafluids_amd.solver import Solver
     s = Solver(cfg)
-    s.init_vortex_tubes(K.CONFIG2_SEED, 1.0)
+    s.init_vortex_tubes(K.VORTEX_TANK_SEED, 1.0)
     return s
 
 
@@ -282,7 +282,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic: seeded divergence-free vortex tubes (seed 1), sphere inflow",
-        "config": {"workload": f"config 2: 3-D {n}^3 MAC grid, FP64, cubic B-spline fields, "
+        "config": {"workload": f"3-D {n}^3 vortex tank: MAC grid, FP64, cubic B-spline fields, "
                                f"RK3, Jacobi-PCG to 1e-8, dt clamped to {dt_eff:.6g}",
                    "grid": [n, n, n], "halo": cfg["halo"], "cg_tol": cfg["cg_tol"]},
         "pcg": {"iterations_per_step": round(total_iters / args.steps, 1),
@@ -292,7 +292,7 @@ def main():
                 "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv)
                                 for kk, vv in v.items()} for k, v in per.items()},
                 "max_final_residual": max(res64)},
-        "advect": {"avg_ms": round(adv_ms / adv_n, 3) if adv_n else None},
+        "advect": advect_figures(cfg, adv_ms / adv_n) if adv_n else None,
         "roofline": roofline,
     }
 
@@ -336,8 +336,79 @@ def main():
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, n, total_iters / args.steps, args.cpu_iters)
+    out["dye_tank_128"] = dye_tank_leg(not args.no_cpu)
 
     print(json.dumps(out))
+
+
+# FP64 operations (add, sub and mul 1, fma 2) per advected point of a 3-D
+# step with cubic fields and linear velocity, counted from the device code:
+# RK3 = 3 velocity evaluations x 3 components x (12 for the weights + 28 for
+# the 8-tap product sum) + 39 for the stage positions and the combination =
+# 399; the cubic field sample = 66 for the weights + 208 for the 64-tap sum.
+ADVECT_FLOPS_PER_POINT_3D_CUBIC = 399 + 274
+FP64_PEAK_TFLOPS = 78.6
+
+
+def advect_figures(cfg, avg_ms):
+    """Advection's rates: owned points (cells and faces), unique HBM bytes
+    (read each state array once, write each Next array once) and FP64
+    operations per launch over its mean duration."""
+    from cajitafluids_amd.solver import layout
+    pts = 0
+    for e in range(cfg["dim"] + 1):
+        _, lo, hi = layout(cfg, e)
+        pts += (hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2])
+    out = {"avg_ms": round(avg_ms, 3), "points": pts,
+           "gbs_unique": round(16 * pts / (avg_ms * 1e-3) / 1e9, 1)}
+    if cfg["dim"] == 3 and cfg["field_order"] == 3 and cfg["velocity_order"] == 1:
+        tf = ADVECT_FLOPS_PER_POINT_3D_CUBIC * pts / (avg_ms * 1e-3) / 1e12
+        out.update(flops_per_point=ADVECT_FLOPS_PER_POINT_3D_CUBIC,
+                   tflops=round(tf, 2), fp64_peak_frac=round(tf / FP64_PEAK_TFLOPS, 4),
+                   hbm_frac=round(out["gbs_unique"] / HBM_PEAK_GBS, 4),
+                   bound="gathers: 136 taps per point through L1/L2, three RK3 stages in sequence")
+    return out
+
+
+def dye_tank_leg(with_cpu):
+    """The 2-D 128^2 dye tank, setup + 50 steps (linear fields, Jacobi-PCG
+    to 1e-8), as a whole run: on the GPU, and with the oracle on the host
+    cores.  At 128^2 the GPU path is launch- and latency-bound."""
+    from cajitafluids_amd import configs as K
+    from cajitafluids_amd.solver import Solver
+    cfg = K.dye_tank_2d(128)
+    nsteps = K.DYE_TANK_STEPS
+    cells = 128 * 128
+    out = {"workload": "2-D 128^2 dye tank, setup + 50 steps, FP64, linear fields",
+           "unit": "cell-steps/s"}
+    with Solver(cfg) as s:
+        s.setup()          # warm-up run (module load, first launches)
+        s.sync()
+    with Solver(cfg) as s:
+        t0 = time.perf_counter()
+        s.setup()
+        its = 0
+        for _ in range(nsteps):
+            its += s.step()["iterations"]
+        s.sync()
+        t = time.perf_counter() - t0
+    out["gpu"] = {"value": cells * nsteps / t, "seconds": round(t, 4),
+                  "pcg_iterations": its}
+    if with_cpu:
+        sys.path.insert(0, os.path.join(REPO, "tests"))
+        import oracle_lib as O
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        orc = O.Oracle(cfg)
+        t0 = time.perf_counter()
+        rc, _ = orc.setup()
+        for _ in range(nsteps):
+            rc, _ = orc.step()
+        t = time.perf_counter() - t0
+        orc.close()
+        out["cpu_oracle"] = {"value": cells * nsteps / t, "seconds": round(t, 3),
+                             "cores": threads, "kind": "port"}
+        out["gpu_over_cpu"] = round(out["gpu"]["value"] / out["cpu_oracle"]["value"], 2)
+    return out
 
 
 def mg_leg(n, args, value64, q64, p64):
@@ -439,9 +510,9 @@ def adjoint_legs():
     c4 = K.adjoint_tank_2d(512)
     from cajitafluids_amd.solver import Solver
     with Solver(c4) as s:
-        s.init_blobs(K.CONFIG4_TARGET_SEED, 8)
+        s.init_blobs(K.DYE_ADJOINT_TARGET_SEED, 8)
         target = s.get(0)
-    rng = np.random.default_rng(K.CONFIG4_FD_SEED)
+    rng = np.random.default_rng(K.DYE_ADJOINT_FD_SEED)
     faces = {}
     for e in (1, 2):
         from cajitafluids_amd.solver import layout
@@ -454,9 +525,9 @@ def adjoint_legs():
     def init4(s):
         for e, a in faces.items():
             s.set(e, a)
-    out["dye_tank_512"] = adjoint_leg(c4, K.CONFIG4_STEPS, K.CONFIG4_CKPT, init4,
+    out["dye_tank_512"] = adjoint_leg(c4, K.DYE_ADJOINT_STEPS, K.DYE_ADJOINT_CKPT, init4,
                                       lambda s: s.get(0) - target)
-    out["dye_tank_512_mg"] = adjoint_leg(with_mg(c4), K.CONFIG4_STEPS, K.CONFIG4_CKPT, init4,
+    out["dye_tank_512_mg"] = adjoint_leg(with_mg(c4), K.DYE_ADJOINT_STEPS, K.DYE_ADJOINT_CKPT, init4,
                                          lambda s: s.get(0) - target)
     compare_mg(out, "dye_tank_512")
     # 3-D 384^3 periodic box: vortex tubes (seed 5), dye blobs (seed 55),
@@ -464,11 +535,11 @@ def adjoint_legs():
     c5 = K.periodic_box_3d(384)
 
     def init5(s):
-        s.init_vortex_tubes(K.CONFIG5_VEL_SEED, 1.0)
-        s.init_blobs(K.CONFIG5_DYE_SEED, 8)
-    out["periodic_box_384"] = adjoint_leg(c5, K.CONFIG5_STEPS, K.CONFIG5_CKPT, init5,
+        s.init_vortex_tubes(K.PERIODIC_BOX_VEL_SEED, 1.0)
+        s.init_blobs(K.PERIODIC_BOX_DYE_SEED, 8)
+    out["periodic_box_384"] = adjoint_leg(c5, K.PERIODIC_BOX_STEPS, K.PERIODIC_BOX_CKPT, init5,
                                           lambda s: s.get(0))
-    out["periodic_box_384_mg"] = adjoint_leg(with_mg(c5), K.CONFIG5_STEPS, K.CONFIG5_CKPT,
+    out["periodic_box_384_mg"] = adjoint_leg(with_mg(c5), K.PERIODIC_BOX_STEPS, K.PERIODIC_BOX_CKPT,
                                              init5, lambda s: s.get(0))
     compare_mg(out, "periodic_box_384")
     return out
@@ -521,7 +592,7 @@ def cpu_baseline(cfg, n, iters_per_step, cpu_iters):
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     sol = Solver(cfg)
-    sol.init_vortex_tubes(K.CONFIG2_SEED, 1.0)
+    sol.init_vortex_tubes(K.VORTEX_TANK_SEED, 1.0)
     sol.setup()
     c2 = dict(cfg)
     c2["cg_max_iter"] = cpu_iters

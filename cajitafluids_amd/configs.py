@@ -56,7 +56,7 @@ def base_config(dim=2, n=128):
 
 
 def dye_tank_2d(n=128, field_order=1):
-    """Config 1: 2-D n x n MAC grid on the unit square, FP64, dt = 0.005
+    """The 2-D dye tank: n x n MAC grid on the unit square, FP64, dt = 0.005
     (clamped by the reference rule to dx / 3), linear field sampling, zero
     initial state, inflow box [0.45, 0.55] x [0.20, 0.22] with q = 1 and
     v = 3, SOLID walls, Jacobi-PCG to 1e-8; 50 RK3 steps."""
@@ -72,11 +72,11 @@ def dye_tank_2d(n=128, field_order=1):
     return c
 
 
-CONFIG1_STEPS = 50
+DYE_TANK_STEPS = 50
 
 
 def vortex_tank_3d(n=256, precision=FP64):
-    """Config 2: 3-D n^3 MAC grid, FP64, cubic field sampling, dt = 0.002
+    """The 3-D vortex tank: n^3 MAC grid, FP64, cubic field sampling, dt = 0.002
     (clamped to dx / 2 = 1/512 at n = 256), initial velocity = seeded
     divergence-free vortex tubes (seed 1, peak |u| = 1), inflow sphere
     r = 0.05 at (0.5, 0.5, 0.1) with q = 1 and w = 2, SOLID walls,
@@ -95,12 +95,12 @@ def vortex_tank_3d(n=256, precision=FP64):
     return c
 
 
-CONFIG2_STEPS = 20
-CONFIG2_SEED = 1
+VORTEX_TANK_STEPS = 20
+VORTEX_TANK_SEED = 1
 
 
 def random_solve_3d(n=512):
-    """Config 3: pressure solve only on an n^3 grid.  Face velocities i.i.d.
+    """The random-velocity pressure solve on an n^3 grid (no step).  Face velocities i.i.d.
     N(0,1) (seed 3), RHS = -div / dx with its mean removed; FP32 inner CG to
     1e-4 inside FP64 defect correction to 1e-8 (each pass restarts the inner
     CG).  `refine_mode=REFINE_CARRY, inner_tol=1e-2` is the faster variant
@@ -111,11 +111,11 @@ def random_solve_3d(n=512):
     return c
 
 
-CONFIG3_SEED = 3
+RANDOM_SOLVE_SEED = 3
 
 
 def adjoint_tank_2d(n=512):
-    """Config 4: 2-D n^2, FP64, linear fields, config 1's inflow and walls,
+    """The 2-D dye-tank adjoint: n^2, FP64, linear fields, the dye tank's inflow and walls,
     dt held at the clamp value for speed 3 (dx / 3); 100 steps; loss
     0.5 ||q_T - target||^2 with target = 8 seeded Gaussian blobs (seed 4);
     checkpoint every 10 steps."""
@@ -124,14 +124,14 @@ def adjoint_tank_2d(n=512):
     return c
 
 
-CONFIG4_STEPS = 100
-CONFIG4_CKPT = 10
-CONFIG4_TARGET_SEED = 4
-CONFIG4_FD_SEED = 44
+DYE_ADJOINT_STEPS = 100
+DYE_ADJOINT_CKPT = 10
+DYE_ADJOINT_TARGET_SEED = 4
+DYE_ADJOINT_FD_SEED = 44
 
 
 def periodic_box_3d(n=384):
-    """Config 5: 3-D n^3 single-rank subdomain with 2-cell ghosts, periodic
+    """The 3-D periodic box: n^3 single-rank subdomain with 2-cell ghosts, periodic
     in x and y, SOLID walls in z, cubic fields, FP32-refined PCG to 1e-8,
     vortex tubes (seed 5), dye = seeded Gaussian blobs (seed 55); 10-step
     adjoint checkpointed every 2 steps."""
@@ -147,10 +147,10 @@ def periodic_box_3d(n=384):
     return c
 
 
-CONFIG5_STEPS = 10
-CONFIG5_CKPT = 2
-CONFIG5_VEL_SEED = 5
-CONFIG5_DYE_SEED = 55
+PERIODIC_BOX_STEPS = 10
+PERIODIC_BOX_CKPT = 2
+PERIODIC_BOX_VEL_SEED = 5
+PERIODIC_BOX_DYE_SEED = 55
 
 
 def copy_config(c):

@@ -251,10 +251,10 @@ cf_status cf_pressure_view(cf_solver* h, int which, cf_view* out)
     });
 }
 
-// Dense ghosted host array <-> padded device layout, one 2-D copy per plane.
-// One contiguous host transfer through a dense device buffer, repacked on
-// the device (plane-by-plane 2-D copies from pageable memory ran at about
-// 3 ms per plane).
+// Dense ghosted host array <-> padded device layout: one contiguous host
+// transfer through a dense device buffer, repacked on the device
+// (plane-by-plane 2-D copies from pageable memory ran at about 3 ms per
+// plane).
 static void copy_dense(cfmi::Solver& s, double* dev, int entity, void* host,
                        bool to_device)
 {

@@ -32,7 +32,7 @@ NSAMPLE = 64
 
 
 def workload():
-    return K.adjoint_tank_2d(N), K.CONFIG4_STEPS, K.CONFIG4_CKPT
+    return K.adjoint_tank_2d(N), K.DYE_ADJOINT_STEPS, K.DYE_ADJOINT_CKPT
 
 
 def initial_state(orc):
@@ -68,7 +68,7 @@ def sample_indices(orc, entity):
 def main():
     cfg, nsteps, ckpt = workload()
     tgt = O.Oracle(cfg)
-    target = tgt.init_blobs(K.CONFIG4_TARGET_SEED, 8)
+    target = tgt.init_blobs(K.DYE_ADJOINT_TARGET_SEED, 8)
     tv = tgt.owned_view(target, 0).copy()
     tgt.close()
 
@@ -98,7 +98,7 @@ def main():
         "workload": "2-D 512^2 dye tank, setup + 100 steps, checkpoint every 10, "
                     "loss 0.5 ||q_T - blobs||^2",
         "init_seed": INIT_SEED, "sample_seed": SAMPLE_SEED,
-        "target_seed": K.CONFIG4_TARGET_SEED, "nblobs": 8,
+        "target_seed": K.DYE_ADJOINT_TARGET_SEED, "nblobs": 8,
         "nsteps": nsteps, "ckpt": ckpt, "cg_tol": cfg["cg_tol"],
         "loss": loss,
         "dvel": [float(x) for x in g["dvel"]],

@@ -301,7 +301,7 @@ def test_dye_tank_fd_32_coordinates(cf):
     cfg, nsteps, ckpt = G.workload()
     cfg["cg_tol"] = 1e-14
     with cf.Solver(cfg) as s:
-        s.init_blobs(K.CONFIG4_TARGET_SEED, 8)
+        s.init_blobs(K.DYE_ADJOINT_TARGET_SEED, 8)
         tv = _owned(cf, cfg, 0, s.get(0))
     ext = {e: cf.layout(cfg, e)[0] for e in range(3)}
 
@@ -329,7 +329,7 @@ def test_dye_tank_fd_32_coordinates(cf):
     _, lo, hi = cf.layout(cfg, 0)
     dqT[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]] = qT - tv
     g, _ = _gpu_adjoint(cf, cfg, init, nsteps, ckpt, dqT)
-    rng = np.random.default_rng(K.CONFIG4_FD_SEED)
+    rng = np.random.default_rng(K.DYE_ADJOINT_FD_SEED)
     eps = 1e-6
     worst = 0.0
     for n in range(32):

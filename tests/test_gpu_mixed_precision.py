@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 
 def _solve(cf, cfg, precision):
     with cf.Solver(cfg) as sol:
-        sol.init_random_faces(K.CONFIG3_SEED)
+        sol.init_random_faces(K.RANDOM_SOLVE_SEED)
         sol.build_rhs()
         sol.rhs_remove_mean()
         st = sol.pressure_solve(precision)

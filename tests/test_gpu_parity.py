@@ -109,7 +109,7 @@ def test_pcg_3d_ragged_tiles(cf, oracle_mod, periodic, precision):
     assert rel_l2(x_g, x_o) <= TOL
 
 
-def test_config1_small_steps(cf, oracle_mod):
+def test_dye_tank_small_steps(cf, oracle_mod):
     cfg = small_2d(32)
     sol, orc = _pair(cf, oracle_mod, cfg)
     assert sol.dt == orc.dt
@@ -124,24 +124,24 @@ def test_config1_small_steps(cf, oracle_mod):
         _assert_state(sol, orc, what=f"step {n}")
 
 
-def test_config1_full(cf, oracle_mod):
-    """Config 1 exactly: 2-D 128^2, linear fields, 50 RK3 steps."""
+def test_dye_tank_128_full(cf, oracle_mod):
+    """The 2-D dye tank exactly: 128^2, linear fields, 50 RK3 steps."""
     cfg = K.dye_tank_2d(128)
     sol, orc = _pair(cf, oracle_mod, cfg)
     sol.setup()
     orc.setup()
     its_g, its_o = [], []
-    for _ in range(K.CONFIG1_STEPS):
+    for _ in range(K.DYE_TANK_STEPS):
         its_g.append(sol.step()["iterations"])
         rc, so = orc.step()
         assert rc == 0
         its_o.append(so["iterations"])
     assert its_g == its_o
-    _assert_state(sol, orc, what="config1 final")
+    _assert_state(sol, orc, what="dye tank 128^2 final")
 
 
-def test_config1_cubic_as_written(cf, oracle_mod):
-    """The as-written field order (cubic) on config 1's setup at 64^2."""
+def test_dye_tank_cubic_as_written(cf, oracle_mod):
+    """The as-written field order (cubic) on the dye tank's setup at 64^2."""
     cfg = small_2d(64, field_order=3)
     sol, orc = _pair(cf, oracle_mod, cfg)
     sol.setup()
@@ -156,7 +156,7 @@ def test_config1_cubic_as_written(cf, oracle_mod):
 def test_3d_vortex_steps(cf, oracle_mod, quirks):
     cfg = small_3d(16, quirks=quirks)
     sol, orc = _pair(cf, oracle_mod, cfg)
-    sol.init_vortex_tubes(K.CONFIG2_SEED, 1.0)
+    sol.init_vortex_tubes(K.VORTEX_TANK_SEED, 1.0)
     gpu_to_oracle(sol, orc)
     sol.setup()
     orc.setup()
@@ -169,12 +169,12 @@ def test_3d_vortex_steps(cf, oracle_mod, quirks):
 
 
 def test_3d_periodic_fp32_refined(cf, oracle_mod):
-    """Config 5's shape at 16^3: h = 2, periodic x/y, walls in z, cubic,
+    """The periodic box's shape at 16^3: h = 2, periodic x/y, walls in z, cubic,
     FP32-refined pressure, vortex tubes + blob dye."""
     cfg = K.periodic_box_3d(16)
     sol, orc = _pair(cf, oracle_mod, cfg)
-    sol.init_vortex_tubes(K.CONFIG5_VEL_SEED, 1.0)
-    sol.init_blobs(K.CONFIG5_DYE_SEED, 8)
+    sol.init_vortex_tubes(K.PERIODIC_BOX_VEL_SEED, 1.0)
+    sol.init_blobs(K.PERIODIC_BOX_DYE_SEED, 8)
     gpu_to_oracle(sol, orc)
     sol.setup()
     orc.setup()
