@@ -60,15 +60,6 @@ void copy_arrays(const Geo& g, Fields dst, Fields src, hipStream_t s)
                               hipMemcpyDeviceToDevice, s));
 }
 
-// A has a constant null space unless some wall is FREE (Dirichlet).
-bool singular_operator(const Geo& g)
-{
-    for (int d = 0; d < g.dim; ++d)
-        if (!g.periodic[d] && (!g.solid[d][0] || !g.solid[d][1]))
-            return false;
-    return true;
-}
-
 // sum of diag(A) over the owned cells: the wall count of a cell is the sum of
 // independent per-axis counts, so its distribution is a convolution.
 double diag_sum(const Geo& g)
@@ -108,7 +99,7 @@ cf_status Solver::adjointRun(int nsteps, int ckpt, const double* dq_T,
     const int nck = nsteps / ckpt + 1;
     const int seglen = nsteps < ckpt ? (nsteps > 0 ? nsteps : 1) : ckpt;
     const double time0 = _time;
-    const bool singular = singular_operator(g);
+    const bool singular = operator_singular(g);
     const double dsum = diag_sum(g);
     const RedBuf rb = _vc->redBuf();
     double* ysum = _vc->scratch() + 4;
@@ -131,7 +122,12 @@ cf_status Solver::adjointRun(int nsteps, int ckpt, const double* dq_T,
     auto project_adj = [&](Fields b) -> cf_status {
         launch_adj_mask(g, b, _s);
         launch_adj_gradT(g, b, y->ptr, ysum, rb, _s);
-        if (singular)
+        // Jacobi-PCG's map kills diag(A) 1, so the shift follows diag(A); the
+        // MG-PCG solve returns the zero-mean pressure, whose transpose removes
+        // the mean of y first
+        if (singular && _cfg.preconditioner == CF_PRECOND_MG)
+            launch_rhs_shift(g, y->ptr, ysum, _s);
+        else if (singular)
             launch_adj_shift(g, y->ptr, ysum, dsum, _s);
         cf_solve_stats ss;
         cf_status rc = _vc->solve(_cfg.precision, y->ptr, z->ptr, &ss);

@@ -60,6 +60,15 @@ struct Geo
     }
 };
 
+// Whether A has the constant null space: no FREE wall on a non-periodic axis.
+inline bool operator_singular(const Geo& g)
+{
+    for (int d = 0; d < g.dim; ++d)
+        if (!g.periodic[d] && (!g.solid[d][0] || !g.solid[d][1]))
+            return false;
+    return true;
+}
+
 // Fills every field of g from the config; returns CF_OK or CF_EINVAL and a
 // message.  Host only.
 cf_status build_geometry(const cf_config& c, Geo& g, const char** why);

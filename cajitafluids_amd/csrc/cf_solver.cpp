@@ -487,6 +487,15 @@ cf_status VelocityCorrector::solveMg(const double* b, double* x, cf_solve_stats*
         _timer.resolve(hs.done ? hs.iter + 1 : INT_MAX);
         batch = 2;
     }
+    // On a singular operator the pressure is fixed up to a constant, which
+    // MG-PCG leaves at a level of the V-cycle's (only the as-written FaceJ
+    // rule lets it reach the velocities).  Return the zero-mean pressure:
+    // that map's transpose is a mean removal and the same solve, which the
+    // reverse mode applies.
+    if (operator_singular(_g)) {
+        launch_rhs_sum(_g, x, _scratch.ptr + 1, rb, _s);
+        launch_rhs_shift(_g, x, _scratch.ptr + 1, _s);
+    }
     _predict_mg = hs.iter;
     if (st) {
         st->converged = hs.converged;

@@ -518,7 +518,7 @@ void cfo_build_rhs(cfo_state* s)
                    s->rhs);
 }
 
-void cfo_rhs_remove_mean(cfo_state* s)
+void cfo_remove_mean(cfo_state* s, double* v)
 {
     int lo[3], hi[3];
     cfo_owned_range(s, 0, lo, hi);
@@ -527,7 +527,7 @@ void cfo_rhs_remove_mean(cfo_state* s)
     for (int k = lo[2]; k < hi[2]; ++k)
         for (int j = lo[1]; j < hi[1]; ++j)
             for (int i = lo[0]; i < hi[0]; ++i) {
-                cfo_dd_add(&acc, s->rhs[cfo_idx(s, 0, i, j, k)], 0.0);
+                cfo_dd_add(&acc, v[cfo_idx(s, 0, i, j, k)], 0.0);
                 ++cnt;
             }
     double mean = cfo_dd_value(acc) / (double)cnt;
@@ -535,8 +535,19 @@ void cfo_rhs_remove_mean(cfo_state* s)
         for (int j = lo[1]; j < hi[1]; ++j)
             for (int i = lo[0]; i < hi[0]; ++i) {
                 long c = cfo_idx(s, 0, i, j, k);
-                s->rhs[c] = s->rhs[c] - mean;
+                v[c] = v[c] - mean;
             }
+}
+
+void cfo_rhs_remove_mean(cfo_state* s) { cfo_remove_mean(s, s->rhs); }
+
+int cfo_operator_singular(const struct cfo_state* s)
+{
+    for (int d = 0; d < s->dim; ++d)
+        if (!s->c.periodic[d] && (s->c.wall_type[d][0] == CFO_FREE ||
+                                  s->c.wall_type[d][1] == CFO_FREE))
+            return 0;
+    return 1;
 }
 
 void cfo_apply_pressure_arrays(struct cfo_state* s, double* u, double* v,

@@ -27,7 +27,10 @@
  * singular (no FREE wall: constant null space) the adjoint right-hand side
  * y is first made consistent by y -= diag(A) * sum(y) / sum(diag(A)), which
  * is what L does to the null-space component; without the projection quirk
- * sum(y) is zero up to rounding and the shift changes nothing.
+ * sum(y) is zero up to rounding and the shift changes nothing.  The MG-PCG
+ * solve returns the zero-mean pressure on a singular operator, (I - 1 1^T /
+ * n) L_B b, whose transpose is L_B (I - 1 1^T / n): there the shift is the
+ * plain mean removal.
  */
 #include "cf_oracle_internal.h"
 
@@ -309,15 +312,6 @@ static void inputs_adj(const struct cfo_state* s, double* const* pre,
     }
 }
 
-static int singular_operator(const struct cfo_state* s)
-{
-    for (int d = 0; d < s->dim; ++d)
-        if (!s->c.periodic[d] && (s->c.wall_type[d][0] == CFO_FREE ||
-                                  s->c.wall_type[d][1] == CFO_FREE))
-            return 0;
-    return 1;
-}
-
 /* project^T (VelocityCorrector.hpp:182-264 with the gathers of the forward) */
 static int project_adj(struct cfo_state* s, double** bar, double* y, double* z)
 {
@@ -361,7 +355,9 @@ static int project_adj(struct cfo_state* s, double** bar, double* y, double* z)
     /* 3. consistency shift on a singular operator, then z = L y */
     int lo[3], hi[3];
     cfo_owned_range(s, 0, lo, hi);
-    if (singular_operator(s)) {
+    if (cfo_operator_singular(s) && s->c.preconditioner == CFO_PRECOND_MG) {
+        cfo_remove_mean(s, y);
+    } else if (cfo_operator_singular(s)) {
         cfo_dd sy = {0, 0}, sd = {0, 0};
         for (int k = lo[2]; k < hi[2]; ++k)
             for (int j = lo[1]; j < hi[1]; ++j)

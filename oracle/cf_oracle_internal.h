@@ -73,6 +73,10 @@ static inline double cfo_coord(const struct cfo_state* s, int e, int d, int l)
 }
 
 void cfo_owned_range(const struct cfo_state* s, int e, int lo[3], int hi[3]);
+/* 1 when A has the constant null space (no FREE wall on a non-periodic axis) */
+int cfo_operator_singular(const struct cfo_state* s);
+/* v -= mean of v over the owned cells (double-double sum) */
+void cfo_remove_mean(struct cfo_state* s, double* v);
 void cfo_fill_ghosts_array(struct cfo_state* s, int e, double* a);
 void cfo_advect_entity(struct cfo_state* s, int e, const double* const* cur,
                        double* next);

@@ -413,6 +413,16 @@ int cfo_pcg_mg(struct cfo_state* s, const double* b, double* x, double tol, int 
         CFO_FOR_OWNED_CELLS(s, p[c] = fma(beta, p[c], z[c]));
         rho = rho_new;
     }
+    /* On a singular operator the pressure is fixed up to a constant, and
+     * MG-PCG leaves it at a level of its own, B^1/2 (B^1/2 A B^1/2)^+ B^1/2 b
+     * for the V-cycle B.  Only the as-written FaceJ rule (T3) lets that
+     * level reach the velocities.  The solve returns the zero-mean pressure
+     * instead, so that the reverse mode can transpose the map exactly:
+     * (I - 1 1^T / n) L_B has the transpose L_B (I - 1 1^T / n), a mean
+     * removal followed by the same MG-PCG, whereas L_B's own null vector,
+     * B^-1 1, is not at hand. */
+    if (cfo_operator_singular(s))
+        cfo_remove_mean(s, x);
     if (st) {
         st->converged = conv;
         st->iterations = k;

@@ -119,6 +119,10 @@ def _case(name):
                  inflow_velocity=[0.3, -0.2, 2.0], inflow_quantity=1.0,
                  clamp_dt=0, dt=0.2)
         return c, 3, 2
+    if name == "2d_linear_quirks_mg":
+        c, n, k = _case("2d_linear_quirks")
+        c["preconditioner"] = K.PRECOND_MG
+        return c, n, k
     raise KeyError(name)
 
 
@@ -188,7 +192,8 @@ def test_adjoint_checkpoint_invariance(cf):
             assert rel_l2(g[k], ref[k]) <= 1e-10, (ck, k)
 
 
-@pytest.mark.parametrize("name", ["2d_linear_quirks", "3d_periodic_xy_cubic"])
+@pytest.mark.parametrize("name", ["2d_linear_quirks", "3d_periodic_xy_cubic",
+                                  "2d_linear_quirks_mg"])
 def test_adjoint_vs_finite_differences(cf, name):
     # The forward map is piecewise smooth: a linear B-spline sample changes
     # slope where a trace crosses a knot, and the PCG's stopping iteration
