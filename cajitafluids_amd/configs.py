@@ -12,7 +12,7 @@ SOLID, FREE = 0, 1
 INFLOW_NONE, INFLOW_BOX, INFLOW_SPHERE = 0, 1, 2
 FP64, FP32_REFINED = 0, 1
 REFINE_RESTART, REFINE_CARRY = 0, 1
-PRECOND_JACOBI, PRECOND_MG = 0, 1
+PRECOND_JACOBI, PRECOND_MG, PRECOND_NONE = 0, 1, 2
 QUIRK_RK3_V0 = 1      # TimeIntegrator.hpp:57-58 (3/4 probe uses v0)
 QUIRK_PROJ_BC_U = 2   # VelocityCorrector.hpp:260 (FaceJ BC applied to u)
 

@@ -123,9 +123,9 @@ cf_status Solver::adjointRun(int nsteps, int ckpt, const double* dq_T,
         launch_adj_mask(g, b, _s);
         launch_adj_gradT(g, b, y->ptr, ysum, rb, _s);
         // Jacobi-PCG's map kills diag(A) 1, so the shift follows diag(A); the
-        // MG-PCG solve returns the zero-mean pressure, whose transpose removes
-        // the mean of y first
-        if (singular && _cfg.preconditioner == CF_PRECOND_MG)
+        // MG-PCG and unpreconditioned solves return the zero-mean pressure,
+        // whose transpose removes the mean of y first
+        if (singular && _cfg.preconditioner != CF_PRECOND_JACOBI)
             launch_rhs_shift(g, y->ptr, ysum, _s);
         else if (singular)
             launch_adj_shift(g, y->ptr, ysum, dsum, _s);

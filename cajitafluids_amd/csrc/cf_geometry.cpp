@@ -52,8 +52,9 @@ cf_status build_geometry(const cf_config& c, Geo& g, const char** why)
         *why = "cg_tol and inner_tol must be positive, max_outer >= 0";
         return CF_EINVAL;
     }
-    if (c.preconditioner != CF_PRECOND_JACOBI && c.preconditioner != CF_PRECOND_MG) {
-        *why = "preconditioner must be CF_PRECOND_JACOBI or CF_PRECOND_MG";
+    if (c.preconditioner != CF_PRECOND_JACOBI && c.preconditioner != CF_PRECOND_MG &&
+        c.preconditioner != CF_PRECOND_NONE) {
+        *why = "preconditioner must be CF_PRECOND_JACOBI, CF_PRECOND_MG or CF_PRECOND_NONE";
         return CF_EINVAL;
     }
     if (c.preconditioner == CF_PRECOND_MG) {

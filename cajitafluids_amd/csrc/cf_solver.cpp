@@ -277,10 +277,10 @@ int VelocityCorrector::runIterations(T* x, T* r, DevArray<T> (&p)[2], int& cur,
         for (int c = 0; c < batch; ++c) {
             const int it = it0 + launched + c + 1;
             int t0 = _timer.begin_iter(_s, it);
-            launch_pcg_dir_spmv<T>(_g, r, p[cur].ptr, p[cur ^ 1].ptr, _st, rb, _s);
+            launch_pcg_dir_spmv<T>(pcgGeo(), r, p[cur].ptr, p[cur ^ 1].ptr, _st, rb, _s);
             _timer.end(t0, _s, tk0 + 0, it);
             int t2 = _timer.begin_iter(_s, it);
-            launch_pcg_update<T>(_g, x, r, p[cur ^ 1].ptr, _st, rb, _s);
+            launch_pcg_update<T>(pcgGeo(), x, r, p[cur ^ 1].ptr, _st, rb, _s);
             _timer.end(t2, _s, tk0 + 1, it);
             cur ^= 1;
         }
@@ -301,7 +301,7 @@ int VelocityCorrector::runPcg64(const double* b, double* x, double tol,
                                 int maxit)
 {
     RedBuf rb{_red_partials.ptr, _red_counter.ptr};
-    launch_pcg_init64(_g, b, x, _r.ptr, _st, tol, maxit, rb, _s);
+    launch_pcg_init64(pcgGeo(), b, x, _r.ptr, _st, tol, maxit, rb, _s);
     int cur = 0;
     const int its =
         runIterations<double>(x, _r.ptr, _p, cur, _predict64, TK_DIR_SPMV);
@@ -343,7 +343,7 @@ cf_status VelocityCorrector::solveRestart(const double* b, double* x,
                                           cf_solve_stats& loc)
 {
     RedBuf rb{_red_partials.ptr, _red_counter.ptr};
-    launch_refine_init(_g, b, x, _r32.ptr, _rs, rb, _s);
+    launch_refine_init(pcgGeo(), b, x, _r32.ptr, _rs, rb, _s);
     readRefine();
     loc.b_norm = _rs_host->bnorm;
     if (_rs_host->bnorm == 0.0) {
@@ -362,7 +362,7 @@ cf_status VelocityCorrector::solveRestart(const double* b, double* x,
         if (loc.outer_iterations >= _cfg.max_outer ||
             loc.iterations >= _cfg.cg_max_iter)
             return CF_ENOCONV;
-        launch_pcg_init32(_g, _r32.ptr, _d32.ptr, _st, _rs, _cfg.inner_tol,
+        launch_pcg_init32(pcgGeo(), _r32.ptr, _d32.ptr, _st, _rs, _cfg.inner_tol,
                           _cfg.cg_tol, inner_max, rb, _s);
         int cur = 0;
         int& pred = passPrediction(loc.outer_iterations);
@@ -370,9 +370,9 @@ cf_status VelocityCorrector::solveRestart(const double* b, double* x,
             runIterations<float>(_d32.ptr, _r32.ptr, _p32, cur, pred, TK_DIR_SPMV32);
         pred = its;
         loc.iterations += its;
-        launch_refine_add(_g, x, _d32.ptr, _s);
+        launch_refine_add(pcgGeo(), x, _d32.ptr, _s);
         int t = _timer.begin(_s);
-        launch_refine_residual(_g, b, x, _r32.ptr, _rs, rb, _s);
+        launch_refine_residual(pcgGeo(), b, x, _r32.ptr, _rs, rb, _s);
         _timer.end(t, _s, TK_RESIDUAL, 0);
         readRefine();
         _timer.resolve(INT_MAX);
@@ -386,7 +386,7 @@ cf_status VelocityCorrector::solveCarry(const double* b, double* x,
                                         cf_solve_stats& loc)
 {
     RedBuf rb{_red_partials.ptr, _red_counter.ptr};
-    launch_refine_init_carry(_g, b, x, _r32.ptr, _d32.ptr, _rs, _st,
+    launch_refine_init_carry(pcgGeo(), b, x, _r32.ptr, _d32.ptr, _rs, _st,
                              _cfg.inner_tol, _cfg.cg_tol, _cfg.cg_max_iter, rb, _s);
     readRefine();
     loc.b_norm = _rs_host->bnorm;
@@ -406,9 +406,9 @@ cf_status VelocityCorrector::solveCarry(const double* b, double* x,
         loc.iterations = its;
         if (!_st_host->converged) // iteration cap reached inside the pass
             return CF_ENOCONV;
-        launch_refine_add(_g, x, _d32.ptr, _s);
+        launch_refine_add(pcgGeo(), x, _d32.ptr, _s);
         int t = _timer.begin(_s);
-        launch_refine_residual_carry(_g, b, x, _r32.ptr, _rs, _st, _cfg.inner_tol,
+        launch_refine_residual_carry(pcgGeo(), b, x, _r32.ptr, _rs, _st, _cfg.inner_tol,
                                      _cfg.cg_tol, rb, _s);
         _timer.end(t, _s, TK_RESIDUAL, 0);
         readRefine();
@@ -440,13 +440,31 @@ cf_status VelocityCorrector::solveRefined(const double* b, double* x,
 cf_status VelocityCorrector::solve(int precision, const double* b, double* x,
                                    cf_solve_stats* st)
 {
-    if (precision == CF_FP32_REFINED)
-        return solveRefined(b, x, st);
+    if (precision == CF_FP32_REFINED) {
+        const cf_status rc = solveRefined(b, x, st);
+        zeroMeanIfUnpreconditioned(x);
+        return rc;
+    }
     if (precision != CF_FP64)
         throw CfError(CF_EINVAL, "precision must be CF_FP64 or CF_FP32_REFINED");
     if (_cfg.preconditioner == CF_PRECOND_MG)
         return solveMg(b, x, st);
-    return solve64(b, x, st);
+    const cf_status rc = solve64(b, x, st);
+    zeroMeanIfUnpreconditioned(x);
+    return rc;
+}
+
+// Without a preconditioner, CG from x = 0 stays in the range of A: its map
+// is A^+ and the solution has zero mean up to rounding.  On a singular
+// operator the mean is removed explicitly, as after the multigrid solve, so
+// that the map's transpose is exactly a mean removal and the same solve.
+void VelocityCorrector::zeroMeanIfUnpreconditioned(double* x)
+{
+    if (_cfg.preconditioner != CF_PRECOND_NONE || !operator_singular(_g))
+        return;
+    RedBuf rb{_red_partials.ptr, _red_counter.ptr};
+    launch_rhs_sum(_g, x, _scratch.ptr + 1, rb, _s);
+    launch_rhs_shift(_g, x, _scratch.ptr + 1, _s);
 }
 
 // MG-PCG: Jacobi-PCG's kernels with z = B r from one V-cycle.  The

@@ -190,6 +190,13 @@ class VelocityCorrector
     std::unique_ptr<Multigrid> _mg; // built on the first MG solve
     Geo _g_unit;                    // _g with a unit Jacobi table: the
                                     // direction kernel then reads z as is
+    // geometry of the Jacobi-PCG kernels: _g, or _g_unit without a
+    // preconditioner (z = r)
+    const Geo& pcgGeo() const
+    {
+        return _cfg.preconditioner == CF_PRECOND_NONE ? _g_unit : _g;
+    }
+    void zeroMeanIfUnpreconditioned(double* x);
     std::vector<int> _predict_pass; // FP32 iterations of each refinement pass
 };
 

@@ -14,7 +14,7 @@ import numpy as np
 
 from . import _lib
 from .configs import (CELL, FACE_X, FACE_Y, FACE_Z, FP32_REFINED, FP64,
-                      INFLOW_BOX, INFLOW_NONE, PRECOND_JACOBI, PRECOND_MG,
+                      INFLOW_BOX, INFLOW_NONE, PRECOND_JACOBI, PRECOND_MG, PRECOND_NONE,
                       QUIRK_PROJ_BC_U, QUIRK_RK3_V0, SOLID, base_config)
 
 CURRENT, NEXT = 0, 1
@@ -333,6 +333,11 @@ def create_solver(device: str, global_bounding_box, global_num_cell, density: fl
                                 **options))
 
 
+# HYPRE preconditioner names accepted with matrix solver "PCG"
+PCG_PRECONDITIONERS = {"PFMG": PRECOND_MG, "SMG": PRECOND_MG, "Jacobi": PRECOND_JACOBI,
+                       "Diagonal": PRECOND_JACOBI, "None": PRECOND_NONE, "none": PRECOND_NONE}
+
+
 def solver_config(device: str, global_bounding_box, global_num_cell, density: float,
                   bc: BoundaryCondition, source: Optional[InflowSource],
                   body: Optional[BodyForce], delta_t: float,
@@ -343,20 +348,22 @@ def solver_config(device: str, global_bounding_box, global_num_cell, density: fl
     (src/VelocityCorrector.hpp:297-336): matrix_solver "Reference" is the
     Jacobi-PCG (preconditioner ignored, as there); "PCG" takes the
     preconditioner "PFMG" (the reference example's default,
-    examples/advection.cpp:186-189), served by the library's geometric
-    multigrid V-cycle, or "Jacobi".  Other HYPRE solvers and preconditioners
-    are not provided.  options: any cf_config field (field_order, cg_tol,
-    precision, quirks, periodic, mg_nu, ...)."""
+    examples/advection.cpp:186-189) or "SMG", both served by the library's
+    geometric multigrid V-cycle, "Jacobi" or "Diagonal", or "None" / "none"
+    (plain CG, VelocityCorrector.hpp:327).  Other HYPRE solvers (GMRES,
+    BiCGSTAB, PFMG or SMG as the solver) are not provided.  options: any
+    cf_config field (field_order, cg_tol, precision, quirks, periodic,
+    mg_nu, ...)."""
     if device not in ("hip", "gfx950"):
         raise RuntimeError("invalid backend: only the HIP (gfx950) backend exists")
     if matrix_solver == "Reference":
         precond = PRECOND_JACOBI
-    elif matrix_solver == "PCG" and preconditioner in ("PFMG", "Jacobi"):
-        precond = PRECOND_MG if preconditioner == "PFMG" else PRECOND_JACOBI
+    elif matrix_solver == "PCG" and preconditioner in PCG_PRECONDITIONERS:
+        precond = PCG_PRECONDITIONERS[preconditioner]
     else:
         raise RuntimeError(f"matrix solver {matrix_solver!r} with preconditioner "
                            f"{preconditioner!r} is not provided (Reference, or PCG with "
-                           "PFMG or Jacobi)")
+                           "PFMG, SMG, Jacobi, Diagonal or None)")
     dim = len(global_num_cell)
     cfg = base_config(dim, global_num_cell[0])
     cfg["num_cell"] = list(global_num_cell) + [1] * (3 - dim)

@@ -37,7 +37,7 @@ enum { CF_SOLID = 0, CF_FREE = 1 }; /* BoundaryType (BoundaryConditions.hpp:22-2
 enum { CF_INFLOW_NONE = 0, CF_INFLOW_BOX = 1, CF_INFLOW_SPHERE = 2 };
 enum { CF_FP64 = 0, CF_FP32_REFINED = 1 };
 enum { CF_REFINE_RESTART = 0, CF_REFINE_CARRY = 1 };
-enum { CF_PRECOND_JACOBI = 0, CF_PRECOND_MG = 1 };
+enum { CF_PRECOND_JACOBI = 0, CF_PRECOND_MG = 1, CF_PRECOND_NONE = 2 };
 
 /* As-written reference behaviours, reproduced when set. */
 #define CF_QUIRK_RK3_V0 1    /* 3/4-step probe uses v0: TimeIntegrator.hpp:57-58 */
@@ -86,8 +86,10 @@ typedef struct cf_config {
                              replaced by the FP64 one at each pass) */
     /* Pressure-solve preconditioner (VelocityCorrector.hpp:300-339 names
      * the choice; the reference's default is HYPRE's PCG + PFMG): Jacobi,
-     * or a geometric multigrid V-cycle (FP64 only). */
-    int preconditioner;   /* CF_PRECOND_JACOBI or CF_PRECOND_MG */
+     * a geometric multigrid V-cycle (FP64 only), or none ("None",
+     * VelocityCorrector.hpp:327: plain CG).  With MG or none, the solve
+     * returns the zero-mean pressure on a singular operator. */
+    int preconditioner;   /* CF_PRECOND_JACOBI, CF_PRECOND_MG or CF_PRECOND_NONE */
     int mg_nu;            /* Jacobi sweeps before and after each coarse
                              correction (>= 1) */
     double mg_omega;      /* Jacobi damping in (0, 1] */

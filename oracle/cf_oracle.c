@@ -191,8 +191,9 @@ int cfo_create(const cfo_config* cfg, cfo_state** out)
                             a[0] -= scale;
                     }
                 }
-                /* VelocityCorrector.hpp:174-179: Jacobi M = 1 / A_diag. */
-                s->M[c] = 1.0 / a[0];
+                /* VelocityCorrector.hpp:174-179: Jacobi M = 1 / A_diag;
+                 * the identity for precon "None" (VelocityCorrector.hpp:327) */
+                s->M[c] = cfg->preconditioner == CFO_PRECOND_NONE ? 1.0 : 1.0 / a[0];
             }
 
     *out = s;
@@ -1015,19 +1016,29 @@ int cfo_pressure_solve(cfo_state* s, int precision, cfo_stats* st)
             if (!s->x32 || !s->r32 || !s->z32 || !s->p32 || !s->q32)
                 return CFO_ENOMEM;
         }
-        if (s->c.refine_mode == CFO_REFINE_CARRY)
-            return cfo_pcg_carry(s, s->rhs, s->lhs, st);
-        return cfo_pcg_refined(s, s->rhs, s->lhs, st);
+        int rc = s->c.refine_mode == CFO_REFINE_CARRY
+                     ? cfo_pcg_carry(s, s->rhs, s->lhs, st)
+                     : cfo_pcg_refined(s, s->rhs, s->lhs, st);
+        if (s->c.preconditioner == CFO_PRECOND_NONE && cfo_operator_singular(s))
+            cfo_remove_mean(s, s->lhs);
+        return rc;
     }
     return cfo_solve64(s, s->rhs, s->lhs, s->c.cg_tol, s->c.cg_max_iter, st);
 }
 
+/* Without a preconditioner, CG from x = 0 stays in the range of A, so its
+ * map is A^+ and the solution has zero mean up to rounding; on a singular
+ * operator the mean is removed explicitly, as after the multigrid solve,
+ * so that the map's transpose is exactly a mean removal and the same solve. */
 int cfo_solve64(struct cfo_state* s, const double* b, double* x, double tol,
                 int maxit, cfo_stats* st)
 {
     if (s->c.preconditioner == CFO_PRECOND_MG)
         return cfo_pcg_mg(s, b, x, tol, maxit, st);
-    return cfo_pcg64(s, b, x, tol, maxit, st);
+    int rc = cfo_pcg64(s, b, x, tol, maxit, st);
+    if (s->c.preconditioner == CFO_PRECOND_NONE && cfo_operator_singular(s))
+        cfo_remove_mean(s, x);
+    return rc;
 }
 
 /* VelocityCorrector.hpp:266-282.  A solve that does not converge aborts the

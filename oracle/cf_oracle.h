@@ -85,7 +85,7 @@ typedef struct cfo_config {
     int max_outer;
     int quirks;
     int refine_mode; /* CFO_REFINE_RESTART or CFO_REFINE_CARRY */
-    int preconditioner;   /* CFO_PRECOND_JACOBI or CFO_PRECOND_MG */
+    int preconditioner;   /* CFO_PRECOND_JACOBI, CFO_PRECOND_MG or CFO_PRECOND_NONE */
     int mg_nu;            /* Jacobi sweeps before and after the coarse correction */
     double mg_omega;      /* Jacobi damping */
     int mg_coarse_sweeps; /* Jacobi sweeps on the coarsest level */
@@ -93,7 +93,7 @@ typedef struct cfo_config {
 } cfo_config;
 
 enum { CFO_REFINE_RESTART = 0, CFO_REFINE_CARRY = 1 };
-enum { CFO_PRECOND_JACOBI = 0, CFO_PRECOND_MG = 1 };
+enum { CFO_PRECOND_JACOBI = 0, CFO_PRECOND_MG = 1, CFO_PRECOND_NONE = 2 };
 
 typedef struct cfo_stats {
     int converged;

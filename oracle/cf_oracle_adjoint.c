@@ -30,7 +30,8 @@
  * sum(y) is zero up to rounding and the shift changes nothing.  The MG-PCG
  * solve returns the zero-mean pressure on a singular operator, (I - 1 1^T /
  * n) L_B b, whose transpose is L_B (I - 1 1^T / n): there the shift is the
- * plain mean removal.
+ * plain mean removal, as it is without a preconditioner (map A^+, also
+ * returned at zero mean).
  */
 #include "cf_oracle_internal.h"
 
@@ -355,7 +356,7 @@ static int project_adj(struct cfo_state* s, double** bar, double* y, double* z)
     /* 3. consistency shift on a singular operator, then z = L y */
     int lo[3], hi[3];
     cfo_owned_range(s, 0, lo, hi);
-    if (cfo_operator_singular(s) && s->c.preconditioner == CFO_PRECOND_MG) {
+    if (cfo_operator_singular(s) && s->c.preconditioner != CFO_PRECOND_JACOBI) {
         cfo_remove_mean(s, y);
     } else if (cfo_operator_singular(s)) {
         cfo_dd sy = {0, 0}, sd = {0, 0};

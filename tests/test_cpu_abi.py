@@ -76,7 +76,7 @@ def test_invalid_configs_rejected_before_any_launch(bad):
 
 
 @pytest.mark.parametrize("bad", [
-    dict(preconditioner=2), dict(mg_nu=0), dict(mg_coarse_sweeps=0),
+    dict(preconditioner=3), dict(preconditioner=-1), dict(mg_nu=0), dict(mg_coarse_sweeps=0),
     dict(mg_omega=0.0), dict(mg_omega=1.5), dict(mg_alpha=0.0), dict(mg_alpha=2.5),
 ])
 def test_invalid_multigrid_configs(bad):
@@ -86,6 +86,28 @@ def test_invalid_multigrid_configs(bad):
     with pytest.raises(_lib.CfError) as ei:
         layout(cfg, K.CELL)
     assert ei.value.code == _lib.CF_EINVAL
+
+
+def test_factory_maps_hypre_preconditioner_names():
+    """createVelocityCorrector's names (src/VelocityCorrector.hpp:300-339):
+    PCG with PFMG / SMG -> the V-cycle, Jacobi / Diagonal -> Jacobi,
+    None / none -> plain CG; other HYPRE solvers are refused."""
+    from cajitafluids_amd.solver import BoundaryCondition, solver_config
+    want = {"PFMG": K.PRECOND_MG, "SMG": K.PRECOND_MG, "Jacobi": K.PRECOND_JACOBI,
+            "Diagonal": K.PRECOND_JACOBI, "None": K.PRECOND_NONE, "none": K.PRECOND_NONE}
+    bc = BoundaryCondition()
+    for name, pre in want.items():
+        c = solver_config("hip", [0, 0, 1, 1], [16, 16], 1.0, bc, None, None, 0.01,
+                          "PCG", name)
+        assert c["preconditioner"] == pre, name
+        layout(c, K.CELL)  # accepted by the library's validation
+    c = solver_config("hip", [0, 0, 1, 1], [16, 16], 1.0, bc, None, None, 0.01,
+                      "Reference", "None")
+    assert c["preconditioner"] == K.PRECOND_JACOBI
+    for solver in ("GMRES", "BiCGSTAB", "PFMG"):
+        with pytest.raises(RuntimeError):
+            solver_config("hip", [0, 0, 1, 1], [16, 16], 1.0, bc, None, None, 0.01,
+                          solver, "PFMG")
 
 
 def test_multigrid_fp32_not_implemented():
@@ -115,13 +137,15 @@ def test_create_without_device_fails_cleanly():
 def test_create_solver_choices():
     """createVelocityCorrector's choice (src/VelocityCorrector.hpp:297-336):
     "Reference" is the Jacobi-PCG whatever the preconditioner; "PCG" takes
-    "PFMG" (the example's default) or "Jacobi"; others are refused."""
+    "PFMG" (the example's default) or "Jacobi" (and the names mapped in
+    test_factory_maps_hypre_preconditioner_names); others are refused."""
     from cajitafluids_amd.solver import BoundaryCondition, solver_config
     args = ("hip", [0.0, 0.0, 1.0, 1.0], [16, 16], 0.1, BoundaryCondition(), None, None, 0.005)
     assert solver_config(*args, "Reference", "PFMG")["preconditioner"] == K.PRECOND_JACOBI
     assert solver_config(*args, "PCG", "PFMG")["preconditioner"] == K.PRECOND_MG
     assert solver_config(*args, "PCG", "Jacobi")["preconditioner"] == K.PRECOND_JACOBI
-    for bad in (("GMRES", "PFMG"), ("PCG", "SMG"), ("PCG", "None")):
+    for bad in (("GMRES", "PFMG"), ("BiCGSTAB", "Jacobi"), ("PCG", "Euclid"),
+                ("PCG", "ParaSails")):
         with pytest.raises(RuntimeError):
             solver_config(*args, *bad)
     cfg = solver_config(*args, "PCG", "PFMG", mg_nu=3)

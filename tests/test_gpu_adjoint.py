@@ -119,9 +119,9 @@ def _case(name):
                  inflow_velocity=[0.3, -0.2, 2.0], inflow_quantity=1.0,
                  clamp_dt=0, dt=0.2)
         return c, 3, 2
-    if name == "2d_linear_quirks_mg":
+    if name in ("2d_linear_quirks_mg", "2d_linear_quirks_none"):
         c, n, k = _case("2d_linear_quirks")
-        c["preconditioner"] = K.PRECOND_MG
+        c["preconditioner"] = K.PRECOND_MG if name.endswith("_mg") else K.PRECOND_NONE
         return c, n, k
     raise KeyError(name)
 
@@ -129,7 +129,7 @@ def _case(name):
 CASES = ["2d_linear_quirks", "2d_cubic_cubic", "2d_no_quirks", "2d_periodic_x",
          "2d_periodic_xy_cubic", "2d_free_wall", "3d_periodic_xy_cubic",
          "3d_tank_quirks", "3d_tank_fp32_refined", "3d_tank_high_cfl",
-         "3d_periodic_high_cfl"]
+         "3d_periodic_high_cfl", "2d_linear_quirks_none"]
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -193,7 +193,7 @@ def test_adjoint_checkpoint_invariance(cf):
 
 
 @pytest.mark.parametrize("name", ["2d_linear_quirks", "3d_periodic_xy_cubic",
-                                  "2d_linear_quirks_mg"])
+                                  "2d_linear_quirks_mg", "2d_linear_quirks_none"])
 def test_adjoint_vs_finite_differences(cf, name):
     # The forward map is piecewise smooth: a linear B-spline sample changes
     # slope where a trace crosses a knot, and the PCG's stopping iteration

@@ -168,6 +168,30 @@ def test_3d_vortex_steps(cf, oracle_mod, quirks):
         _assert_state(sol, orc, what=f"3d step {n}")
 
 
+@pytest.mark.parametrize("precision", [K.FP64, K.FP32_REFINED])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_unpreconditioned_steps(cf, oracle_mod, dim, precision):
+    """Preconditioner "None" (plain CG; the zero-mean pressure on a singular
+    operator): setup + 3 steps against the oracle, with the FaceJ quirk on,
+    so that the pressure's level reaches the velocities."""
+    quirks = K.QUIRK_RK3_V0 | K.QUIRK_PROJ_BC_U
+    cfg = small_2d(32, quirks=quirks) if dim == 2 else small_3d(16, quirks=quirks)
+    cfg.update(preconditioner=K.PRECOND_NONE, precision=precision)
+    sol, orc = _pair(cf, oracle_mod, cfg)
+    if dim == 3:
+        sol.init_vortex_tubes(K.VORTEX_TANK_SEED, 1.0)
+        gpu_to_oracle(sol, orc)
+    sol.setup()
+    orc.setup()
+    for n in range(3):
+        st = sol.step()
+        rc, so = orc.step()
+        assert rc == 0 and st["iterations"] == so["iterations"]
+        _assert_state(sol, orc, what=f"none {dim}d step {n}")
+    p_g = sol.owned_view(sol.get_pressure(0), 0)
+    assert abs(float(p_g.mean())) <= 1e-14 * float(np.abs(p_g).max())
+
+
 def test_3d_periodic_fp32_refined(cf, oracle_mod):
     """The periodic box's shape at 16^3: h = 2, periodic x/y, walls in z, cubic,
     FP32-refined pressure, vortex tubes + blob dye."""

@@ -79,6 +79,9 @@ def _cases():
     c = K.copy_config(out["3d_tank_quirks"][0])
     c["preconditioner"] = K.PRECOND_MG
     out["3d_tank_quirks_mg"] = (c, 2, 1)
+    c = K.copy_config(t)
+    c["preconditioner"] = K.PRECOND_NONE
+    out["2d_linear_quirks_none"] = (c, 4, 2)
     return out
 
 

@@ -208,6 +208,30 @@ def test_pcg_matches_pseudoinverse(dim, n, precision):
     assert np.linalg.norm(A @ x - b) <= 2e-12 * np.linalg.norm(b)
 
 
+@pytest.mark.parametrize("dim,n", [(2, 8), (3, 5)])
+@pytest.mark.parametrize("precision", [K.FP64, K.FP32_REFINED])
+def test_unpreconditioned_cg_is_the_pseudoinverse(dim, n, precision):
+    """Preconditioner "None": CG from 0 stays in the range of A, and the
+    solve returns the zero-mean pressure, so on a singular operator it is
+    the minimum-norm solution pinv(A) b itself, with no constant left."""
+    cfg = K.base_config(dim, n)
+    cfg.update(cg_tol=1e-12, clamp_dt=0, dt=0.002, preconditioner=K.PRECOND_NONE)
+    o = O.Oracle(cfg)
+    A, idx = _dense_A(o, dim, n)
+    rng = np.random.default_rng(6)
+    b = rng.standard_normal(len(idx))
+    b -= b.mean()
+    rhs = o.rhs()
+    for v, (k, j, i) in zip(b, idx):
+        rhs[k, j, i] = v
+    rc, st = o.pressure_solve(precision)
+    assert rc == 0 and st["converged"]
+    x = np.array([o.lhs()[k, j, i] for (k, j, i) in idx])
+    ref = np.linalg.pinv(A) @ b
+    assert abs(x.mean()) <= 1e-15 * np.abs(ref).max()
+    assert np.linalg.norm(x - ref) <= 1e-9 * np.linalg.norm(ref)
+
+
 def test_pcg_zero_rhs_and_iteration_cap():
     cfg = K.base_config(2, 8)
     o = O.Oracle(cfg)
