@@ -261,15 +261,10 @@ struct UpdEpiT
                 rb[B][q][0] = load_any<T>(r, ok, at, c.safe);
                 continue;
             }
-#pragma unroll
-            for (int j = 0; j < CPL; ++j) {
-                xb[B][q][j] = (T)0;
-                rb[B][q][j] = (T)0;
-            }
-            if (ok) {
-                vload<T, CPL>(x + at, xb[B][q]);
-                vload<T, CPL>(r + at, rb[B][q]);
-            }
+            // an aligned pair on every path (raw, as above)
+            const int64_t ap = ok ? at : c.safe;
+            vload<T, CPL>(x + ap, xb[B][q]);
+            vload<T, CPL>(r + ap, rb[B][q]);
         }
     }
     __device__ void shift()
@@ -307,7 +302,7 @@ struct UpdEpiT
     }
 };
 
-template <class T, int CPL>
+template <class T, int CPL, bool PX>
 __global__ __launch_bounds__(256) void k_pcg_dir_spmv3(
     Geo g, Tile3Grid tg, const T* __restrict__ r, const T* __restrict__ p_old,
     T* __restrict__ p_new, PcgState* st, RedBuf rb)
@@ -319,13 +314,13 @@ __global__ __launch_bounds__(256) void k_pcg_dir_spmv3(
     const bool first = st->iter == 0;
     const T beta = (T)st->beta;
     DirEpiT<T, CPL> epi{g, p_new};
-    tile3_march<T, true, CPL>(g, tg, r, p_old, first, beta, pl, epi);
+    tile3_march<T, true, CPL, PX>(g, tg, r, p_old, first, beta, pl, epi);
     dd_norm(epi.hi[0], epi.lo[0]);
     grid_reduce<1>(epi.hi, epi.lo, rb,
                    [&](double* sm) { st->alpha = st->rho / sm[0]; });
 }
 
-template <class T, int CPL>
+template <class T, int CPL, bool PX>
 __global__ __launch_bounds__(256) void k_pcg_update3(
     Geo g, Tile3Grid tg, T* __restrict__ x, T* __restrict__ r,
     const T* __restrict__ p, PcgState* st, RedBuf rb)
@@ -335,7 +330,7 @@ __global__ __launch_bounds__(256) void k_pcg_update3(
         return;
     __shared__ T pl[2][TY + 2][tx<CPL>() + 2];
     UpdEpiT<T, CPL> epi{g, x, r, (T)st->alpha};
-    tile3_march<T, false, CPL>(g, tg, p, nullptr, false, (T)0, pl, epi);
+    tile3_march<T, false, CPL, PX>(g, tg, p, nullptr, false, (T)0, pl, epi);
     dd_norm(epi.hi[0], epi.lo[0]);
     dd_norm(epi.hi[1], epi.lo[1]);
     grid_reduce<2>(epi.hi, epi.lo, rb, [&](double* sm) {
@@ -564,28 +559,56 @@ void launch_refine_residual_carry(const Geo& g, const double* b,
                          itol, gtol, rb);
 }
 
+// The two-cell (FP32) tile kernels come in a periodic-x instantiation,
+// which also reads the wrapped first cell of an odd row, and a plain one.
+template <class T, bool PX>
+static void dir_spmv3_t(const Geo& g, const T* r, const T* p_old, T* p_new, PcgState* st,
+                        RedBuf rb, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_pcg_dir_spmv3<T, cpl_of<T>(), PX>),
+                       dim3(tile3_blocks<cpl_of<T>()>(g)), dim3(kBlock), 0, s, g,
+                       tile3_grid<cpl_of<T>()>(g), r, p_old, p_new, st, rb);
+}
+
+template <class T, bool PX>
+static void update3_t(const Geo& g, T* x, T* r, const T* p, PcgState* st, RedBuf rb,
+                      hipStream_t s)
+{
+    hipLaunchKernelGGL((k_pcg_update3<T, cpl_of<T>(), PX>),
+                       dim3(tile3_blocks<cpl_of<T>()>(g)), dim3(kBlock), 0, s, g,
+                       tile3_grid<cpl_of<T>()>(g), x, r, p, st, rb);
+}
+
 template <class T>
 void launch_pcg_dir_spmv(const Geo& g, const T* r, const T* p_old, T* p_new,
                          PcgState* st, RedBuf rb, hipStream_t s)
 {
-    if (g.dim == 2)
+    if (g.dim == 2) {
         CF_STREAM_LAUNCH((k_pcg_dir_spmv<2, T>), g, r, p_old, p_new, st, rb);
-    else
-        hipLaunchKernelGGL((k_pcg_dir_spmv3<T, cpl_of<T>()>),
-                           dim3(tile3_blocks<cpl_of<T>()>(g)), dim3(kBlock), 0, s, g,
-                           tile3_grid<cpl_of<T>()>(g), r, p_old, p_new, st, rb);
+        return;
+    }
+    if constexpr (cpl_of<T>() > 1)
+        if (g.periodic[0]) {
+            dir_spmv3_t<T, true>(g, r, p_old, p_new, st, rb, s);
+            return;
+        }
+    dir_spmv3_t<T, false>(g, r, p_old, p_new, st, rb, s);
 }
 
 template <class T>
 void launch_pcg_update(const Geo& g, T* x, T* r, const T* p, PcgState* st,
                        RedBuf rb, hipStream_t s)
 {
-    if (g.dim == 2)
+    if (g.dim == 2) {
         CF_STREAM_LAUNCH((k_pcg_update<2, T>), g, x, r, p, st, rb);
-    else
-        hipLaunchKernelGGL((k_pcg_update3<T, cpl_of<T>()>),
-                           dim3(tile3_blocks<cpl_of<T>()>(g)), dim3(kBlock), 0, s, g,
-                           tile3_grid<cpl_of<T>()>(g), x, r, p, st, rb);
+        return;
+    }
+    if constexpr (cpl_of<T>() > 1)
+        if (g.periodic[0]) {
+            update3_t<T, true>(g, x, r, p, st, rb, s);
+            return;
+        }
+    update3_t<T, false>(g, x, r, p, st, rb, s);
 }
 
 int reduction_blocks_max(const Geo& g)

@@ -21,8 +21,8 @@
 // before touching the current one.  The loaded registers are kept raw and
 // masked where they are used: a select right after the load may be placed
 // at the end of the loop body, where it waits for the loads just as well.
-// FP32 groups of two cells (CPL = 2) still branch around their wrap and
-// x-halo cases.
+// Groups of two cells (CPL = 2, FP32) read aligned pairs on every path too,
+// with the periodic-x wrap compiled in only where x is periodic (PX).
 //
 // Everything that depends on the lane's x position is computed once per
 // block, everything that depends on a row once per block and wave (rows are
@@ -375,7 +375,7 @@ struct Plane3
     T b[tile3::NS][CPL];
 };
 
-template <class T, int MODE, int CPL>
+template <class T, int MODE, int CPL, bool PX = false>
 __device__ __forceinline__ void tile3_load(const Tile3Ctx<CPL>& c, const Plane3Meta& m,
                                            const T* __restrict__ va,
                                            const T* __restrict__ vb, bool need_b,
@@ -387,14 +387,8 @@ __device__ __forceinline__ void tile3_load(const Tile3Ctx<CPL>& c, const Plane3M
     for (int s = 0; s < tile3::NS; ++s) {
         const bool ok = c.colb[s] >= 0 && m.off >= 0;
         const int64_t at = c.colb[s] + m.off;
-#pragma unroll
-        for (int j = 0; j < CPL; ++j) {
-            P.a[s][j] = (T)0;
-            if constexpr (DIR)
-                P.b[s][j] = (T)0;
-        }
         // the halo slot of an x-halo lane is one cell; every other slot is a
-        // lane's group of CPL cells, read as one access unless it wraps
+        // lane's group of CPL cells
         const bool single = CPL > 1 && s == tile3::RY && c.hkind == 2;
         const bool split = CPL > 1 && (s < tile3::RY ? c.split : c.hsplit);
         if constexpr (MODE == 2) {
@@ -410,24 +404,34 @@ __device__ __forceinline__ void tile3_load(const Tile3Ctx<CPL>& c, const Plane3M
             if constexpr (DIR)
                 P.b[s][0] = load_any<T>(vb, ok, at, c.safe);
             continue;
-        }
-        if (ok && !single && !split) {
-            vload<T, CPL>(va + at, P.a[s]);
+        } else {
+            // two cells per lane: one pair access per stream on every path.
+            // Pairs start at even indices (rows start at multiples of 16), so
+            // an x-halo lane reads its cell with the pair neighbour; a zero
+            // ghost reads the block's safe pair (masked where used).  On a
+            // periodic x axis (PX) the last group of an odd row takes its
+            // second cell from the row's start, read on every path too.
+            const int64_t ap = ok ? (at & ~(int64_t)1) : c.safe;
+            const bool odd = (at & 1) != 0;
+            T a2[2], b2[2];
+            vload<T, 2>(va + ap, a2);
             if constexpr (DIR)
-                if (need_b)
-                    vload<T, CPL>(vb + at, P.b[s]);
-        } else if (ok) {
-            P.a[s][0] = va[at];
-            if constexpr (DIR)
-                if (need_b)
-                    P.b[s][0] = vb[at];
-            if (split) {
+                vload<T, 2>(vb + ap, b2);
+            P.a[s][0] = single && odd ? a2[1] : a2[0];
+            P.a[s][1] = single ? (T)0 : a2[1];
+            if constexpr (DIR) {
+                P.b[s][0] = single && odd ? b2[1] : b2[0];
+                P.b[s][1] = single ? (T)0 : b2[1];
+            }
+            if constexpr (PX) {
                 // the row's first cell: n0 - 1 cells back from the last
-                const int64_t at0 = at - c.wrapback;
-                P.a[s][CPL - 1] = va[at0];
-                if constexpr (DIR)
-                    if (need_b)
-                        P.b[s][CPL - 1] = vb[at0];
+                const int64_t aw = ok && split ? at - c.wrapback : c.safe;
+                const T wa = va[aw];
+                P.a[s][1] = split ? wa : P.a[s][1];
+                if constexpr (DIR) {
+                    const T wb = vb[aw];
+                    P.b[s][1] = split ? wb : P.b[s][1];
+                }
             }
         }
     }
@@ -456,10 +460,8 @@ __device__ __forceinline__ T tile3_value(const Geo& g, const Tile3Ctx<CPL>& c,
         const T z = dinv * P.a[s][j];
         v = first ? z : fma_t<T>(beta, P.b[s][j], z);
     }
-    // CPL = 1 loads are raw: a zero ghost slot read some owned cell
-    if constexpr (CPL == 1)
-        return c.colb[s] >= 0 && m.off >= 0 ? v : (T)0;
-    return v;
+    // loads are raw: a zero ghost slot read some owned cell
+    return c.colb[s] >= 0 && m.off >= 0 ? v : (T)0;
 }
 
 // Writes the plane's values into LDS plane pl and own[]; the halo slot
@@ -536,7 +538,7 @@ __device__ __forceinline__ T tile3_apply(const Geo& g, const Tile3Ctx<CPL>& c,
 // B (k >= k1: values never used), take_raw<B>(P) takes the marched operand
 // instead, cells<B>() uses buffer B, shift() copies buffer 1 into 0 (CPL = 2
 // only), rotate() runs after every step's barrier.
-template <class T, int DIR, int CPL, class Epi>
+template <class T, int DIR, int CPL, bool PX = false, class Epi>
 __device__ __forceinline__ void tile3_march(
     const Geo& g, const Tile3Grid& tg, const T* __restrict__ va,
     const T* __restrict__ vb, bool first, T beta,
@@ -557,17 +559,17 @@ __device__ __forceinline__ void tile3_march(
     T pm[RY][CPL], pc[RY][CPL], pp[RY][CPL];
     // prologue: plane k0-1 (own column only), plane k0 into LDS, k0+1 in flight
     Plane3Meta mm = plane3_meta(g, c.k0 - 1);
-    tile3_load<T, DIR, CPL>(c, mm, va, vb, need_b, par, P0);
+    tile3_load<T, DIR, CPL, PX>(c, mm, va, vb, need_b, par, P0);
 #pragma unroll
     for (int q = 0; q < RY; ++q)
 #pragma unroll
         for (int j = 0; j < CPL; ++j)
             pm[q][j] = tile3_value<T, DIR, CPL>(g, c, mm, P0, q, j, first, beta);
     Plane3Meta mc = plane3_meta(g, c.k0);
-    tile3_load<T, DIR, CPL>(c, mc, va, vb, need_b, par, P1);
+    tile3_load<T, DIR, CPL, PX>(c, mc, va, vb, need_b, par, P1);
     tile3_finish<T, DIR, CPL>(g, c, mc, P1, first, beta, pl[0], pc);
     Plane3Meta mn = plane3_meta(g, c.k0 + 1);
-    tile3_load<T, DIR, CPL>(c, mn, va, vb, need_b, par, P0);
+    tile3_load<T, DIR, CPL, PX>(c, mn, va, vb, need_b, par, P0);
     if constexpr (epi_takes_raw<E>::value)
         epi.template take_raw<0>(P1);
     epi.template prefetch<0>(c, mc, c.k0);
@@ -580,7 +582,7 @@ __device__ __forceinline__ void tile3_march(
         // the last step loads plane k1 again (an L2 hit, never used) rather
         // than branching around the loads
         const Plane3Meta mnn = plane3_meta(g, min(k + 2, c.k1));
-        tile3_load<T, DIR, CPL>(c, mnn, va, vb, need_b, par, Pl);
+        tile3_load<T, DIR, CPL, PX>(c, mnn, va, vb, need_b, par, Pl);
         tile3_finish<T, DIR, CPL>(g, c, mn, Pf, first, beta, pl[S ^ 1], pp);
         if constexpr (epi_takes_raw<E>::value)
             epi.template take_raw<S ^ 1>(Pf);
@@ -618,9 +620,8 @@ __device__ __forceinline__ void tile3_march(
     } else {
         int cur = 0;
         for (int k = c.k0; k < c.k1; ++k) {
-            const Plane3Meta mnn = plane3_meta(g, k + 2);
-            if (k + 2 <= c.k1)
-                tile3_load<T, DIR, CPL>(c, mnn, va, vb, need_b, par, P1);
+            const Plane3Meta mnn = plane3_meta(g, min(k + 2, c.k1));
+            tile3_load<T, DIR, CPL, PX>(c, mnn, va, vb, need_b, par, P1);
             tile3_finish<T, DIR, CPL>(g, c, mn, P0, first, beta, pl[cur ^ 1], pp);
             epi.template prefetch<1>(c, mn, k + 1);
 #pragma unroll
