@@ -31,6 +31,7 @@
 #include "cf_stream.hpp"
 #include "cf_tile3.hpp"
 
+
 namespace cfmi
 {
 
@@ -253,7 +254,7 @@ struct UpdEpiT
 #pragma unroll
         for (int q = 0; q < tile3::RY; ++q) {
             // a group with a valid first cell lies inside its padded row
-            const bool ok = k < c.k1 && c.valid[q][0];
+            const bool ok = k >= c.k0 && k < c.k1 && c.valid[q][0];
             const int64_t at = c.colb[q] + m.off;
             if constexpr (CPL == 1) {
                 // raw: cells() reads only valid rows of planes below k1
@@ -330,7 +331,10 @@ __global__ __launch_bounds__(256) void k_pcg_update3(
         return;
     __shared__ T pl[2][TY + 2][tx<CPL>() + 2];
     UpdEpiT<T, CPL> epi{g, x, r, (T)st->alpha};
-    tile3_march<T, false, CPL, PX>(g, tg, p, nullptr, false, (T)0, pl, epi);
+    // downwards: the direction kernel marches up, so each kernel starts on
+    // the planes the other touched last (Infinity Cache hits; FP64 256^3:
+    // update 127 -> 121 us, direction 101 -> 95 us)
+    tile3_march<T, false, CPL, PX, true>(g, tg, p, nullptr, false, (T)0, pl, epi);
     dd_norm(epi.hi[0], epi.lo[0]);
     dd_norm(epi.hi[1], epi.lo[1]);
     grid_reduce<2>(epi.hi, epi.lo, rb, [&](double* sm) {

@@ -528,7 +528,8 @@ __device__ __forceinline__ T tile3_apply(const Geo& g, const Tile3Ctx<CPL>& c,
 //
 // With one cell per lane the loop is unrolled by two with ping-pong
 // buffers, so no register copy of a plane's loads (which would wait for
-// them) happens before their use: at step s (plane k = k0 + s) the march's
+// them) happens before their use: at step s (plane k = k0 + s; with REV,
+// k1 - 1 - s, and "k+1" below means the next plane down) the march's
 // plane k+1 is in P[s & 1], plane k+2 loads into P[(s + 1) & 1], and the
 // epilogue's data of plane k sits in its buffer s & 1 while plane k+1's
 // loads into (s + 1) & 1.  With two cells per lane (FP32) the second set of
@@ -538,7 +539,12 @@ __device__ __forceinline__ T tile3_apply(const Geo& g, const Tile3Ctx<CPL>& c,
 // B (k >= k1: values never used), take_raw<B>(P) takes the marched operand
 // instead, cells<B>() uses buffer B, shift() copies buffer 1 into 0 (CPL = 2
 // only), rotate() runs after every step's barrier.
-template <class T, int DIR, int CPL, bool PX = false, class Epi>
+//
+// REV marches the chunk downwards (k1-1 to k0).  A kernel that follows one
+// marching the other way then starts on the planes its predecessor touched
+// last, which are still in the Infinity Cache.  The stencil sees the same
+// operands in the same order either way.
+template <class T, int DIR, int CPL, bool PX = false, bool REV = false, class Epi>
 __device__ __forceinline__ void tile3_march(
     const Geo& g, const Tile3Grid& tg, const T* __restrict__ va,
     const T* __restrict__ vb, bool first, T beta,
@@ -556,37 +562,43 @@ __device__ __forceinline__ void tile3_march(
     if (vb == nullptr)
         vb = va;
     Plane3<T, DIR, CPL> P0, P1;
+    // pm: the plane behind the march, pc: the current one, pp: the one ahead
     T pm[RY][CPL], pc[RY][CPL], pp[RY][CPL];
-    // prologue: plane k0-1 (own column only), plane k0 into LDS, k0+1 in flight
-    Plane3Meta mm = plane3_meta(g, c.k0 - 1);
+    constexpr int D = REV ? -1 : 1;
+    const int kbeg = REV ? c.k1 - 1 : c.k0;
+    // the plane two steps ahead, clamped to the plane just past the chunk
+    auto ahead2 = [&](int k) { return REV ? max(k - 2, c.k0 - 1) : min(k + 2, c.k1); };
+    // prologue: the plane behind kbeg (own column only), kbeg into LDS, the
+    // plane ahead in flight
+    Plane3Meta mm = plane3_meta(g, kbeg - D);
     tile3_load<T, DIR, CPL, PX>(c, mm, va, vb, need_b, par, P0);
 #pragma unroll
     for (int q = 0; q < RY; ++q)
 #pragma unroll
         for (int j = 0; j < CPL; ++j)
             pm[q][j] = tile3_value<T, DIR, CPL>(g, c, mm, P0, q, j, first, beta);
-    Plane3Meta mc = plane3_meta(g, c.k0);
+    Plane3Meta mc = plane3_meta(g, kbeg);
     tile3_load<T, DIR, CPL, PX>(c, mc, va, vb, need_b, par, P1);
     tile3_finish<T, DIR, CPL>(g, c, mc, P1, first, beta, pl[0], pc);
-    Plane3Meta mn = plane3_meta(g, c.k0 + 1);
+    Plane3Meta mn = plane3_meta(g, kbeg + D);
     tile3_load<T, DIR, CPL, PX>(c, mn, va, vb, need_b, par, P0);
     if constexpr (epi_takes_raw<E>::value)
         epi.template take_raw<0>(P1);
-    epi.template prefetch<0>(c, mc, c.k0);
+    epi.template prefetch<0>(c, mc, kbeg);
     epi.rotate();
     __syncthreads();
     auto step = [&](int k, auto parity) {
         constexpr int S = decltype(parity)::value;
-        Plane3<T, DIR, CPL>& Pf = S == 0 ? P0 : P1; // plane k+1
-        Plane3<T, DIR, CPL>& Pl = S == 0 ? P1 : P0; // plane k+2, started here
-        // the last step loads plane k1 again (an L2 hit, never used) rather
-        // than branching around the loads
-        const Plane3Meta mnn = plane3_meta(g, min(k + 2, c.k1));
+        Plane3<T, DIR, CPL>& Pf = S == 0 ? P0 : P1; // plane k+D
+        Plane3<T, DIR, CPL>& Pl = S == 0 ? P1 : P0; // plane k+2D, started here
+        // the last step loads the plane past the chunk again (an L2 hit,
+        // never used) rather than branching around the loads
+        const Plane3Meta mnn = plane3_meta(g, ahead2(k));
         tile3_load<T, DIR, CPL, PX>(c, mnn, va, vb, need_b, par, Pl);
         tile3_finish<T, DIR, CPL>(g, c, mn, Pf, first, beta, pl[S ^ 1], pp);
         if constexpr (epi_takes_raw<E>::value)
             epi.template take_raw<S ^ 1>(Pf);
-        epi.template prefetch<S ^ 1>(c, mn, k + 1);
+        epi.template prefetch<S ^ 1>(c, mn, k + D);
 #pragma unroll
         for (int q = 0; q < RY; ++q) {
             // a group with no valid cell has nothing to compute or store
@@ -594,8 +606,9 @@ __device__ __forceinline__ void tile3_march(
                 T y[CPL];
 #pragma unroll
                 for (int j = 0; j < CPL; ++j)
-                    y[j] = tile3_apply<T, CPL>(g, c, mc, pl[S], q, j, pc[q], pm[q][j],
-                                               pp[q][j]);
+                    y[j] = tile3_apply<T, CPL>(g, c, mc, pl[S], q, j, pc[q],
+                                               REV ? pp[q][j] : pm[q][j],
+                                               REV ? pm[q][j] : pp[q][j]);
                 epi.template cells<S>(c, mc, q, pc[q], y);
             }
 #pragma unroll
@@ -610,28 +623,29 @@ __device__ __forceinline__ void tile3_march(
         epi.rotate();
     };
     if constexpr (CPL == 1) {
-        int k = c.k0;
-        for (; k + 1 < c.k1; k += 2) {
+        int k = kbeg, left = c.k1 - c.k0;
+        for (; left >= 2; left -= 2, k += 2 * D) {
             step(k, std::integral_constant<int, 0>{});
-            step(k + 1, std::integral_constant<int, 1>{});
+            step(k + D, std::integral_constant<int, 1>{});
         }
-        if (k < c.k1)
+        if (left > 0)
             step(k, std::integral_constant<int, 0>{});
     } else {
         int cur = 0;
-        for (int k = c.k0; k < c.k1; ++k) {
-            const Plane3Meta mnn = plane3_meta(g, min(k + 2, c.k1));
+        for (int s = 0, k = kbeg; s < c.k1 - c.k0; ++s, k += D) {
+            const Plane3Meta mnn = plane3_meta(g, ahead2(k));
             tile3_load<T, DIR, CPL, PX>(c, mnn, va, vb, need_b, par, P1);
             tile3_finish<T, DIR, CPL>(g, c, mn, P0, first, beta, pl[cur ^ 1], pp);
-            epi.template prefetch<1>(c, mn, k + 1);
+            epi.template prefetch<1>(c, mn, k + D);
 #pragma unroll
             for (int q = 0; q < RY; ++q) {
                 if (c.valid[q][0]) {
                     T y[CPL];
 #pragma unroll
                     for (int j = 0; j < CPL; ++j)
-                        y[j] = tile3_apply<T, CPL>(g, c, mc, pl[cur], q, j, pc[q], pm[q][j],
-                                                   pp[q][j]);
+                        y[j] = tile3_apply<T, CPL>(g, c, mc, pl[cur], q, j, pc[q],
+                                                   REV ? pp[q][j] : pm[q][j],
+                                                   REV ? pm[q][j] : pp[q][j]);
                     epi.template cells<0>(c, mc, q, pc[q], y);
                 }
 #pragma unroll
