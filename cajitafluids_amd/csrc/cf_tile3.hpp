@@ -536,7 +536,7 @@ __device__ __forceinline__ T tile3_apply(const Geo& g, const Tile3Ctx<CPL>& c,
 // live buffers costs more registers than the overlap gains: that loop keeps
 // one buffer pair and copies (P0 = P1, epi.shift()) after each plane.
 // Epilogue interface: prefetch<B>(c, m, k) loads plane k's data into buffer
-// B (k >= k1: values never used), take_raw<B>(P) takes the marched operand
+// B (k outside [k0, k1): values never used), take_raw<B>(P) takes the marched operand
 // instead, cells<B>() uses buffer B, shift() copies buffer 1 into 0 (CPL = 2
 // only), rotate() runs after every step's barrier.
 //
