@@ -85,13 +85,16 @@ def test_pcg_fp32_refined_random_rhs(cf, oracle_mod, dim, n, mode, itol):
 
 @pytest.mark.parametrize("periodic", [[0, 0, 0], [1, 1, 0], [0, 1, 1]])
 @pytest.mark.parametrize("precision", [K.FP64, K.FP32_REFINED])
-def test_pcg_3d_ragged_tiles(cf, oracle_mod, periodic, precision):
-    """Non-cubic 70 x 36 x 40 grid: partial 64 x 16 tiles in x and y and
-    several z chunks, with walls or periodic axes."""
-    n = [70, 36, 40]
+@pytest.mark.parametrize("nz", [40, 33])
+def test_pcg_3d_ragged_tiles(cf, oracle_mod, periodic, precision, nz):
+    """Non-cubic 70 x 36 x nz grid: partial 64 x 16 tiles in x and y and
+    several z chunks, with walls or periodic axes.  nz = 33 ends in a
+    one-plane chunk (16 + 16 + 1 planes for FP64, 4 x 8 + 1 for FP32), the
+    odd tail of both the upward and the downward march."""
+    n = [70, 36, nz]
     cfg = K.base_config(3, 8)
     dx = 1.0 / 70
-    cfg.update(num_cell=n, hi=[1.0, 36 * dx, 40 * dx], clamp_dt=0, dt=0.002,
+    cfg.update(num_cell=n, hi=[1.0, 36 * dx, nz * dx], clamp_dt=0, dt=0.002,
                periodic=periodic, halo=2, precision=precision)
     sol, orc = _pair(cf, oracle_mod, cfg)
     sol.init_random_faces(7)
