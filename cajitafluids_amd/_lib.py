@@ -13,9 +13,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # kernel variants); the default is the in-tree build.
 LIB_PATH = os.environ.get("CFMI_LIBRARY") or os.path.join(HERE, "libcfmi.so")
 
-CF_OK, CF_EINVAL, CF_ENOMEM, CF_ENOCONV, CF_EDEVICE, CF_ENOTIMPL = 0, 1, 2, 3, 4, 5
+CF_OK, CF_EINVAL, CF_ENOMEM, CF_ENOCONV, CF_EDEVICE, CF_ENOTIMPL, CF_ENONFINITE = range(7)
 STATUS_NAMES = {0: "CF_OK", 1: "CF_EINVAL", 2: "CF_ENOMEM", 3: "CF_ENOCONV",
-                4: "CF_EDEVICE", 5: "CF_ENOTIMPL"}
+                4: "CF_EDEVICE", 5: "CF_ENOTIMPL", 6: "CF_ENONFINITE"}
 
 # kernel ids of cf_kernel_time
 (TK_DIR_SPMV, TK_UPDATE, TK_FINALIZE, TK_DIR_SPMV32, TK_UPDATE32, TK_FINALIZE32,
@@ -112,9 +112,12 @@ _P = C.c_void_p
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 _SST = C.POINTER(CfSolveStats)
+_i64p = C.POINTER(C.c_int64)
 SIGNATURES = {
     "cf_layout_query": (C.c_int, [C.POINTER(CfConfig), C.c_int, _ip, _ip, _ip]),
     "cf_geometry_query": (C.c_int, [C.POINTER(CfConfig), _dp, _dp]),
+    "cf_padded_layout": (C.c_int, [C.POINTER(CfConfig), _i64p, _i64p, _i64p, _i64p]),
+    "cf_entity_origin": (C.c_int, [C.POINTER(CfConfig), C.c_int, _dp]),
     "cf_create": (C.c_int, [C.POINTER(CfConfig), C.POINTER(_P)]),
     "cf_destroy": (None, [_P]),
     "cf_last_error": (C.c_char_p, [_P]),
@@ -138,6 +141,8 @@ SIGNATURES = {
     "cf_copy_out": (C.c_int, [_P, C.c_int, C.c_int, _dp]),
     "cf_copy_pressure_in": (C.c_int, [_P, C.c_int, _dp]),
     "cf_copy_pressure_out": (C.c_int, [_P, C.c_int, _dp]),
+    "cf_copy_in_device": (C.c_int, [_P, C.c_int, C.c_int, C.c_void_p]),
+    "cf_copy_out_device": (C.c_int, [_P, C.c_int, C.c_int, C.c_void_p]),
     "cf_advance": (C.c_int, [_P, C.c_int]),
     "cf_fill_ghosts": (C.c_int, [_P, C.c_int]),
     "cf_sync": (C.c_int, [_P]),
@@ -148,6 +153,8 @@ SIGNATURES = {
     "cf_kernel_time": (C.c_int, [_P, C.c_int, _dp, C.POINTER(C.c_int64)]),
     "cf_reset_kernel_time": (C.c_int, [_P]),
     "cf_adjoint_run": (C.c_int, [_P, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "cf_adjoint_run_device": (C.c_int, [_P, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_void_p, _dp]),
     "cf_cell_output": (C.c_int, [_P, _dp, _dp]),
     "cf_write_output": (C.c_int, [_P, C.c_char_p, C.c_int]),
     "cf_write_vti": (C.c_int, [C.c_char_p, C.c_int, _ip, _dp, C.c_double, C.c_int,

@@ -115,8 +115,15 @@ cf_status Solver::adjointRun(int nsteps, int ckpt, const double* dq_T,
     dvel_dev.alloc(4, _s);
 
     cf_solve_stats st;
+    // A failed run leaves the handle as the caller had it: Current holds the
+    // initial state at time0, or the final forward state at time_end once
+    // the forward sweep has finished (the state cf_adjoint_run returns with).
+    bool fwd_done = false;
+    double time_end = time0;
     auto fail = [&](cf_status rc) {
-        _time = time0;
+        copy_arrays(g, _pm->fields(CF_CURRENT), fields_of(g, fwd_done ? fin : init), _s);
+        CF_HIP(hipStreamSynchronize(_s));
+        _time = fwd_done ? time_end : time0;
         return rc;
     };
     auto project_adj = [&](Fields b) -> cf_status {
@@ -151,11 +158,12 @@ cf_status Solver::adjointRun(int nsteps, int ckpt, const double* dq_T,
             return fail(rc);
     }
     copy_arrays(g, fields_of(g, fin), _pm->fields(CF_CURRENT), _s);
-    const double time_end = _time;
+    time_end = _time;
+    fwd_done = true;
 
-    // backward sweep
-    CF_HIP(hipMemcpyAsync(bar.a[0]->ptr, dq_T, (size_t)g.total * sizeof(double),
-                          hipMemcpyDeviceToDevice, _s));
+    // backward sweep (dq_T and the gradients may be caller memory: kernel
+    // copies only)
+    launch_copy_padded(g, CF_CELL, bar.a[0]->ptr, dq_T, true, _s);
     for (int c0 = nsteps > 0 ? ((nsteps - 1) / ckpt) * ckpt : -1; c0 >= 0;
          c0 -= ckpt) {
         const int c1 = c0 + ckpt < nsteps ? c0 + ckpt : nsteps;
@@ -190,9 +198,7 @@ cf_status Solver::adjointRun(int nsteps, int ckpt, const double* dq_T,
 
     for (int e = 0; e < ne; ++e)
         if (grad[e])
-            CF_HIP(hipMemcpyAsync(grad[e], bar.a[e]->ptr,
-                                  (size_t)g.total * sizeof(double),
-                                  hipMemcpyDeviceToDevice, _s));
+            launch_copy_padded(g, e, grad[e], bar.a[e]->ptr, true, _s);
     double hv[4];
     CF_HIP(hipMemcpyAsync(hv, dvel_dev.ptr, sizeof(hv), hipMemcpyDeviceToHost, _s));
     copy_arrays(g, _pm->fields(CF_CURRENT), fields_of(g, fin), _s);

@@ -83,6 +83,43 @@ cf_status cf_geometry_query(const cf_config* cfg, double* dx, double* dt)
     return CF_OK;
 }
 
+cf_status cf_padded_layout(const cf_config* cfg, int64_t* offset, int64_t* pitch_x,
+                           int64_t* pitch_y, int64_t* elems)
+{
+    if (!cfg)
+        return CF_EINVAL;
+    cfmi::Geo g;
+    const char* why;
+    cf_status rc = cfmi::build_geometry(*cfg, g, &why);
+    if (rc != CF_OK)
+        return rc;
+    if (offset)
+        *offset = g.sx;
+    if (pitch_x)
+        *pitch_x = g.px;
+    if (pitch_y)
+        *pitch_y = g.py;
+    if (elems)
+        *elems = g.total;
+    return CF_OK;
+}
+
+cf_status cf_entity_origin(const cf_config* cfg, int entity, double origin[3])
+{
+    if (!cfg || !origin)
+        return CF_EINVAL;
+    cfmi::Geo g;
+    const char* why;
+    cf_status rc = cfmi::build_geometry(*cfg, g, &why);
+    if (rc != CF_OK)
+        return rc;
+    if (!valid_entity(g, entity))
+        return CF_EINVAL;
+    for (int d = 0; d < 3; ++d)
+        origin[d] = d < g.dim ? g.x0[entity][d] : 0.0;
+    return CF_OK;
+}
+
 cf_status cf_create(const cf_config* cfg, cf_solver** out)
 {
     if (!out)
@@ -295,6 +332,30 @@ cf_status cf_copy_out(cf_solver* h, int entity, int version, double* host)
     });
 }
 
+cf_status cf_copy_in_device(cf_solver* h, int entity, int version, const double* dev)
+{
+    return guarded(h, [&](cfmi::Solver& s) {
+        if (!dev || !valid_entity(s.geo(), entity) || (version != 0 && version != 1))
+            throw CfError(CF_EINVAL, "bad entity/version/device pointer");
+        cfmi::launch_copy_padded(s.geo(), entity, s.pm().get(entity, version), dev, false,
+                                 s.stream());
+        CF_HIP(hipGetLastError());
+        return CF_OK;
+    });
+}
+
+cf_status cf_copy_out_device(cf_solver* h, int entity, int version, double* dev)
+{
+    return guarded(h, [&](cfmi::Solver& s) {
+        if (!dev || !valid_entity(s.geo(), entity) || (version != 0 && version != 1))
+            throw CfError(CF_EINVAL, "bad entity/version/device pointer");
+        cfmi::launch_copy_padded(s.geo(), entity, dev, s.pm().get(entity, version), false,
+                                 s.stream());
+        CF_HIP(hipGetLastError());
+        return CF_OK;
+    });
+}
+
 cf_status cf_copy_pressure_in(cf_solver* h, int which, const double* host)
 {
     return guarded(h, [&](cfmi::Solver& s) {
@@ -411,23 +472,11 @@ cf_status cf_adjoint_run(cf_solver* h, int nsteps, int ckpt,
         if (!dq_T_host || !dvel)
             throw CfError(CF_EINVAL, "dq_T and dvel are required");
         const cfmi::Geo& g = s.geo();
-        // dq_T with every non-owned entry cleared: the loss is defined on
-        // the owned cells
-        const int* E = g.ext[CF_CELL];
-        const size_t ncell = (size_t)E[0] * E[1] * E[2];
-        std::vector<double> dq(dq_T_host, dq_T_host + ncell);
-        for (int k = 0; k < E[2]; ++k)
-            for (int j = 0; j < E[1]; ++j)
-                for (int i = 0; i < E[0]; ++i) {
-                    const bool own = i >= g.h && i < g.own_hi[0][0] && j >= g.h &&
-                                     j < g.own_hi[0][1] && k >= g.own_lo(2) &&
-                                     k < g.own_hi[0][2];
-                    if (!own)
-                        dq[((size_t)k * E[1] + j) * E[0] + i] = 0.0;
-                }
+        // the loss is defined on the owned cells: adjointRun ignores the
+        // other entries of dq_T
         cfmi::DevArray<double> dq_dev, gr[4];
         dq_dev.alloc(g.total, s.stream());
-        copy_dense(s, dq_dev.ptr, CF_CELL, dq.data(), true);
+        copy_dense(s, dq_dev.ptr, CF_CELL, (void*)dq_T_host, true);
         double* host_out[4] = {dq0_host, du0_host, dv0_host, dw0_host};
         double* dev_out[4] = {nullptr, nullptr, nullptr, nullptr};
         for (int e = 0; e <= g.dim; ++e)
@@ -442,6 +491,19 @@ cf_status cf_adjoint_run(cf_solver* h, int nsteps, int ckpt,
             if (host_out[e])
                 copy_dense(s, gr[e].ptr, e, host_out[e], false);
         return CF_OK;
+    });
+}
+
+cf_status cf_adjoint_run_device(cf_solver* h, int nsteps, int ckpt,
+                                const double* dq_T_dev, double* du0_dev,
+                                double* dv0_dev, double* dw0_dev, double* dq0_dev,
+                                double dvel[3])
+{
+    return guarded(h, [&](cfmi::Solver& s) -> cf_status {
+        if (!dq_T_dev || !dvel)
+            throw CfError(CF_EINVAL, "dq_T and dvel are required");
+        double* grads[4] = {dq0_dev, du0_dev, dv0_dev, s.geo().dim == 3 ? dw0_dev : nullptr};
+        return s.adjointRun(nsteps, ckpt, dq_T_dev, grads, dvel);
     });
 }
 

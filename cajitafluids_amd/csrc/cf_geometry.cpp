@@ -96,7 +96,7 @@ cf_status build_geometry(const cf_config& c, Geo& g, const char** why)
         for (int s = 0; s < 2; ++s)
             g.solid[d][s] = d < c.dim && c.wall_type[d][s] == CF_SOLID;
     }
-    // Mesh.hpp:472-486: dx from the x extent; every axis must divide.
+    // Mesh.hpp:49-64: dx from the x extent; every axis must divide.
     g.dx = (c.hi[0] - c.lo[0]) / c.num_cell[0];
     for (int d = 0; d < c.dim; ++d) {
         double extent = c.num_cell[d] * g.dx;

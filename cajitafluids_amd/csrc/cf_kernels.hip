@@ -318,6 +318,34 @@ void launch_repack(const Geo& g, int entity, double* padded, double* dense, bool
                        to_padded ? 1 : 0);
 }
 
+__global__ __launch_bounds__(256) void k_copy_padded(Geo g, int e, double* __restrict__ dst,
+                                                    const double* __restrict__ src,
+                                                    int owned_only)
+{
+    const int64_t plane = g.px * g.py;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < g.total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        bool own = true;
+        if (owned_only) {
+            // t = sx + i + px (j + py k); entries before sx or past the last
+            // plane belong to no cell
+            const int64_t u = t - g.sx;
+            const int i = (int)(u % g.px), j = (int)((u / g.px) % g.py);
+            const int64_t k = u / plane;
+            own = u >= 0 && i >= g.h && i < g.own_hi[e][0] && j >= g.own_lo(1) &&
+                  j < g.own_hi[e][1] && k >= g.own_lo(2) && k < g.own_hi[e][2];
+        }
+        dst[t] = own ? src[t] : 0.0;
+    }
+}
+
+void launch_copy_padded(const Geo& g, int entity, double* dst, const double* src,
+                        bool owned_only, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_copy_padded, dim3(2048), dim3(256), 0, s, g, entity, dst, src,
+                       owned_only ? 1 : 0);
+}
+
 void launch_cell_output(const Geo& g, Fields cur, double* q_out, double* vel_out, hipStream_t s)
 {
     dim3 blk(64, 4, 1);

@@ -17,6 +17,7 @@ struct PcgState
     int iter, maxit, done, converged;
     int mg; // z comes from the multigrid V-cycle: the update kernel leaves
             // beta and rho to k_mgpcg_rz
+    int nonfinite; // ||b|| or ||r|| came out NaN or Inf: the solve stopped
 };
 
 // Grid-reduction scratch of the streaming kernels: per-block double-double
@@ -54,6 +55,13 @@ void launch_project(const Geo& g, Fields cur, const double* p, hipStream_t s);
 // Dense ghosted array <-> padded layout (host transfers).
 void launch_repack(const Geo& g, int entity, double* padded, double* dense, bool to_padded,
                    hipStream_t s);
+// A whole padded array copied by a kernel, dst = src over [0, g.total):
+// the device entry points take caller memory that another allocator (another
+// HIP runtime's, in a process that also runs torch) may own, which the
+// runtime's own copy path does not know.  owned_only: entries outside the
+// entity's owned box are written as 0.
+void launch_copy_padded(const Geo& g, int entity, double* dst, const double* src,
+                        bool owned_only, hipStream_t s);
 // The output writer's cell data (owned q, cell-centre velocity; compact).
 void launch_cell_output(const Geo& g, Fields cur, double* q_out, double* vel_out, hipStream_t s);
 

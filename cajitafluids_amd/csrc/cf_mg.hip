@@ -694,10 +694,12 @@ __global__ __launch_bounds__(256) void k_mgpcg_init(Geo g, const double* __restr
         st->alpha = 0.0;
         st->beta = 0.0;
         const bool zero = norm == 0.0;
-        st->done = zero ? 1 : (maxit <= 0 ? 1 : 0);
+        const bool bad = !isfinite(norm); // NaN or Inf in b: stop at once
+        st->done = zero || bad ? 1 : (maxit <= 0 ? 1 : 0);
         st->converged = zero ? 1 : 0;
-        st->res = zero ? 0.0 : 1.0;
+        st->res = zero ? 0.0 : (bad ? norm : 1.0);
         st->mg = 1;
+        st->nonfinite = bad ? 1 : 0;
     });
 }
 

@@ -48,10 +48,38 @@ __device__ __forceinline__ void init_state(PcgState* st, double rz, double nn,
     st->alpha = 0.0;
     st->beta = 0.0;
     const bool zero = norm == 0.0;
-    st->done = zero ? 1 : (maxit <= 0 ? 1 : 0);
+    // a NaN or Inf in the right-hand side stops the solve before it starts
+    const bool bad = !isfinite(norm);
+    st->done = zero || bad ? 1 : (maxit <= 0 ? 1 : 0);
     st->converged = zero ? 1 : 0;
-    st->res = zero ? 0.0 : 1.0;
+    st->res = zero ? 0.0 : (bad ? norm : 1.0);
     st->mg = 0;
+    st->nonfinite = bad ? 1 : 0;
+}
+
+// Scalars after an update: iteration count, relative residual, the stop
+// test, and (Jacobi-PCG) beta and rho.  A residual that is not finite stops
+// the solve at once (the stop test alone would read false and run to maxit).
+__device__ __forceinline__ void finish_update(PcgState* st, double rr, double rz)
+{
+    const int it = st->iter + 1;
+    st->iter = it;
+    const double res = sqrt(rr) / st->bnorm;
+    st->res = res;
+    if (!isfinite(res)) {
+        st->done = 1;
+        st->nonfinite = 1;
+    } else if (res < st->tol) {
+        st->done = 1;
+        st->converged = 1;
+    } else if (it >= st->maxit) {
+        st->done = 1;
+    } else {
+        if (!st->mg) { // MG-PCG: beta and rho follow the V-cycle
+            st->beta = rz / st->rho;
+            st->rho = rz;
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_pcg_init64(
@@ -175,23 +203,7 @@ __global__ __launch_bounds__(256) void k_pcg_update(
             dd_add_prod(hi[1], lo[1], (double)rn, (double)z);
         }
     });
-    grid_reduce<2>(hi, lo, rb, [&](double* sm) {
-        const int it = st->iter + 1;
-        st->iter = it;
-        const double res = sqrt(sm[0]) / st->bnorm;
-        st->res = res;
-        if (res < st->tol) {
-            st->done = 1;
-            st->converged = 1;
-        } else if (it >= st->maxit) {
-            st->done = 1;
-        } else {
-            if (!st->mg) { // MG-PCG: beta and rho follow the V-cycle
-                st->beta = sm[1] / st->rho;
-                st->rho = sm[1];
-            }
-        }
-    });
+    grid_reduce<2>(hi, lo, rb, [&](double* sm) { finish_update(st, sm[0], sm[1]); });
 }
 
 // ---- 3-D kernels (cf_tile3.hpp) ------------------------------------------------
@@ -337,23 +349,8 @@ __global__ __launch_bounds__(256) void k_pcg_update3(
     tile3_march<T, false, CPL, PX, true>(g, tg, p, nullptr, false, (T)0, pl, epi);
     dd_norm(epi.hi[0], epi.lo[0]);
     dd_norm(epi.hi[1], epi.lo[1]);
-    grid_reduce<2>(epi.hi, epi.lo, rb, [&](double* sm) {
-        const int it = st->iter + 1;
-        st->iter = it;
-        const double res = sqrt(sm[0]) / st->bnorm;
-        st->res = res;
-        if (res < st->tol) {
-            st->done = 1;
-            st->converged = 1;
-        } else if (it >= st->maxit) {
-            st->done = 1;
-        } else {
-            if (!st->mg) { // MG-PCG: beta and rho follow the V-cycle
-                st->beta = sm[1] / st->rho;
-                st->rho = sm[1];
-            }
-        }
-    });
+    grid_reduce<2>(epi.hi, epi.lo, rb,
+                   [&](double* sm) { finish_update(st, sm[0], sm[1]); });
 }
 
 // ---- defect correction ---------------------------------------------------------
@@ -450,7 +447,9 @@ __global__ __launch_bounds__(256) void k_refine_init_carry(
         st->alpha = 0.0;
         st->beta = 0.0;
         st->mg = 0;
-        st->done = bnorm == 0.0 ? 1 : (maxit <= 0 ? 1 : 0);
+        // a non-finite ||b|| is reported by the host from rst->bnorm
+        st->nonfinite = isfinite(bnorm) ? 0 : 1;
+        st->done = bnorm == 0.0 || st->nonfinite ? 1 : (maxit <= 0 ? 1 : 0);
         st->converged = bnorm == 0.0 ? 1 : 0;
         if (bnorm == 0.0)
             rst->res = 0.0;

@@ -3,6 +3,7 @@
 #include "cf_solver.hpp"
 
 #include <climits>
+#include <cmath>
 #include <cstring>
 
 #include "cf_init.hpp"
@@ -321,6 +322,8 @@ cf_status VelocityCorrector::solve64(const double* b, double* x,
         st->b_norm = hs.bnorm;
         st->final_residual = hs.res;
     }
+    if (hs.nonfinite)
+        return CF_ENONFINITE;
     return hs.converged ? CF_OK : CF_ENOCONV;
 }
 
@@ -346,6 +349,10 @@ cf_status VelocityCorrector::solveRestart(const double* b, double* x,
     launch_refine_init(pcgGeo(), b, x, _r32.ptr, _rs, rb, _s);
     readRefine();
     loc.b_norm = _rs_host->bnorm;
+    if (!std::isfinite(_rs_host->bnorm)) {
+        loc.final_residual = _rs_host->bnorm;
+        return CF_ENONFINITE;
+    }
     if (_rs_host->bnorm == 0.0) {
         loc.converged = 1;
         loc.final_residual = 0.0;
@@ -370,6 +377,8 @@ cf_status VelocityCorrector::solveRestart(const double* b, double* x,
             runIterations<float>(_d32.ptr, _r32.ptr, _p32, cur, pred, TK_DIR_SPMV32);
         pred = its;
         loc.iterations += its;
+        if (_st_host->nonfinite)
+            return CF_ENONFINITE;
         launch_refine_add(pcgGeo(), x, _d32.ptr, _s);
         int t = _timer.begin(_s);
         launch_refine_residual(pcgGeo(), b, x, _r32.ptr, _rs, rb, _s);
@@ -377,6 +386,10 @@ cf_status VelocityCorrector::solveRestart(const double* b, double* x,
         readRefine();
         _timer.resolve(INT_MAX);
         loc.outer_iterations += 1;
+        if (!std::isfinite(_rs_host->res)) {
+            loc.final_residual = _rs_host->res;
+            return CF_ENONFINITE;
+        }
     }
 }
 
@@ -391,6 +404,10 @@ cf_status VelocityCorrector::solveCarry(const double* b, double* x,
     readRefine();
     loc.b_norm = _rs_host->bnorm;
     loc.final_residual = _rs_host->res;
+    if (!std::isfinite(_rs_host->bnorm)) {
+        loc.final_residual = _rs_host->bnorm;
+        return CF_ENONFINITE;
+    }
     if (_rs_host->bnorm == 0.0) {
         loc.converged = 1;
         return CF_OK;
@@ -404,6 +421,8 @@ cf_status VelocityCorrector::solveCarry(const double* b, double* x,
         pred = its - done_its;
         done_its = its;
         loc.iterations = its;
+        if (_st_host->nonfinite)
+            return CF_ENONFINITE;
         if (!_st_host->converged) // iteration cap reached inside the pass
             return CF_ENOCONV;
         launch_refine_add(pcgGeo(), x, _d32.ptr, _s);
@@ -415,6 +434,8 @@ cf_status VelocityCorrector::solveCarry(const double* b, double* x,
         _timer.resolve(INT_MAX);
         loc.outer_iterations += 1;
         loc.final_residual = _rs_host->res;
+        if (!std::isfinite(_rs_host->res))
+            return CF_ENONFINITE;
         if (_rs_host->res < _cfg.cg_tol) {
             loc.converged = 1;
             return CF_OK;
@@ -522,6 +543,8 @@ cf_status VelocityCorrector::solveMg(const double* b, double* x, cf_solve_stats*
         st->b_norm = hs.bnorm;
         st->final_residual = hs.res;
     }
+    if (hs.nonfinite)
+        return CF_ENONFINITE;
     return hs.converged ? CF_OK : CF_ENOCONV;
 }
 

@@ -84,7 +84,12 @@ __device__ __forceinline__ void grid_reduce(double (&hi)[NV], double (&lo)[NV],
             __hip_atomic_store(&part[2 * v + 1], lo[v], __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         }
-        __builtin_amdgcn_s_waitcnt(0); // partials complete before arrival
+        // partials complete before arrival: the waitcnt orders the hardware;
+        // the signal fence (a compiler-only barrier, no cache maintenance)
+        // keeps the compiler from moving the relaxed stores past the RMW
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
         // Arrivals are counted per group of blocks (blockIdx & 7), each group
         // on its own counter line: same-address atomics serialise at ~11 ns
         // each, so one counter for a 1024-2048-block grid costs 12-24 us.
@@ -105,6 +110,9 @@ __device__ __forceinline__ void grid_reduce(double (&hi)[NV], double (&lo)[NV],
         }
         last = fin_block;
     }
+    // the partial loads below stay after the arrival count (compiler-only
+    // barrier, as above)
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __syncthreads();
     if (!last)
         return;

@@ -39,6 +39,31 @@ def layout(cfg: dict, entity: int):
     return tuple(ext), tuple(lo), tuple(hi)
 
 
+def padded_layout(cfg: dict):
+    """(offset, pitch_x, pitch_y, elems) of the padded device layout every
+    array of a handle uses: element (i, j, k) at offset + i + pitch_x (j +
+    pitch_y k) of an array of elems doubles.  Host-only."""
+    L = _lib.load()
+    st = _lib.config_struct(cfg)
+    vals = [C.c_int64() for _ in range(4)]
+    rc = L.cf_padded_layout(C.byref(st), *(C.byref(v) for v in vals))
+    if rc != 0:
+        raise _lib.CfError(rc, "cf_padded_layout")
+    return tuple(v.value for v in vals)
+
+
+def entity_origin(cfg: dict, entity: int):
+    """Coordinate of local index 0 of an entity (LocalMesh::coordinates at
+    index 0, src/ProblemManager.hpp:202-204).  Host-only."""
+    L = _lib.load()
+    st = _lib.config_struct(cfg)
+    out = (C.c_double * 3)()
+    rc = L.cf_entity_origin(C.byref(st), entity, out)
+    if rc != 0:
+        raise _lib.CfError(rc, "cf_entity_origin")
+    return tuple(out)
+
+
 def geometry(cfg: dict):
     """(dx, effective dt) of a config, host-only."""
     L = _lib.load()
@@ -203,7 +228,10 @@ class Solver:
         return out
 
     def set_pressure(self, arr, which=1):
+        ex = self.extents(CELL)
         a = np.ascontiguousarray(arr, dtype=np.float64)
+        if a.shape != (ex[2], ex[1], ex[0]):
+            raise ValueError(f"expected shape {(ex[2], ex[1], ex[0])}, got {a.shape}")
         self._call(self.L.cf_copy_pressure_in, which,
                    a.ctypes.data_as(C.POINTER(C.c_double)), what="copy_pressure_in")
 
@@ -220,6 +248,20 @@ class Solver:
         v = _lib.CfView()
         self._call(self.L.cf_pressure_view, which, C.byref(v), what="pressure_view")
         return v
+
+    # ---- device-resident state (caller device memory, padded layout) ---------
+    def copy_in_device(self, entity, ptr, version=CURRENT):
+        """State array from a device array of padded_layout(cfg)[3] doubles
+        (every element, ghosts included).  The caller's writes must be
+        complete (synchronise their stream first)."""
+        self._call(self.L.cf_copy_in_device, entity, version, C.c_void_p(int(ptr)),
+                   what="copy_in_device")
+
+    def copy_out_device(self, entity, ptr, version=CURRENT):
+        """State array into a device array in the padded layout; complete
+        after sync()."""
+        self._call(self.L.cf_copy_out_device, entity, version, C.c_void_p(int(ptr)),
+                   what="copy_out_device")
 
     def advance(self, entity):
         self._call(self.L.cf_advance, entity, what="advance")
@@ -285,6 +327,9 @@ class Solver:
         arrays (dense ghosted) and dL/d(inflow velocity)."""
         dqT = np.ascontiguousarray(dq_T, dtype=np.float64)
         ex = {e: self.extents(e) for e in range(self.dim + 1)}
+        if dqT.shape != tuple(ex[CELL][::-1]):
+            raise ValueError(f"dq_T must be the dense ghosted cell array of shape "
+                             f"{tuple(ex[CELL][::-1])}, got {dqT.shape}")
         du = np.zeros(ex[FACE_X][::-1])
         dv = np.zeros(ex[FACE_Y][::-1])
         dw = np.zeros(ex[FACE_Z][::-1]) if self.dim == 3 else None
@@ -294,6 +339,18 @@ class Solver:
         self._call(self.L.cf_adjoint_run, nsteps, ckpt, P(dqT), P(du), P(dv), P(dw),
                    P(dq0), P(dvel), what="adjoint_run")
         return dict(du=du, dv=dv, dw=dw, dq0=dq0, dvel=dvel)
+
+    def adjoint_run_device(self, nsteps, ckpt, dq_T_ptr, du_ptr=None, dv_ptr=None,
+                           dw_ptr=None, dq0_ptr=None):
+        """cf_adjoint_run_device: every array a device pointer in the padded
+        layout (None: not wanted).  Returns dL/d(inflow velocity); the
+        gradient arrays are complete on return."""
+        dvel = np.zeros(3)
+        P = lambda p: None if p is None else C.c_void_p(int(p))
+        self._call(self.L.cf_adjoint_run_device, nsteps, ckpt, P(dq_T_ptr), P(du_ptr),
+                   P(dv_ptr), P(dw_ptr), P(dq0_ptr), dvel.ctypes.data_as(_lib._dp),
+                   what="adjoint_run_device")
+        return dvel
 
 
 # ---- the reference's value objects and factory -----------------------------------

@@ -29,6 +29,8 @@ typedef int cf_status;
 #define CF_ENOCONV 3  /* pressure solve hit its iteration limit */
 #define CF_EDEVICE 4  /* HIP runtime error, or no usable gfx950 device */
 #define CF_ENOTIMPL 5 /* combination not provided by this library */
+#define CF_ENONFINITE 6 /* the pressure solve met a NaN or Inf in its right-hand
+                           side or residual and stopped at once */
 
 /* ---- tags (Cajita entity / ProblemManager version tags) ---------------- */
 enum { CF_CELL = 0, CF_FACE_X = 1, CF_FACE_Y = 2, CF_FACE_Z = 3 };
@@ -51,9 +53,12 @@ enum { CF_PRECOND_JACOBI = 0, CF_PRECOND_MG = 1, CF_PRECOND_NONE = 2 };
  * (src/BoundaryConditions.hpp:131-134; here wall_type[axis][lo/hi]),
  * InflowSource (src/InflowSource.hpp:80-90; box, or a sphere), BodyForce
  * (src/BodyForce.hpp:62-66), density, dt and the pressure-solver choice
- * (src/VelocityCorrector.hpp:300-339; here always Jacobi-PCG).  The initial
- * state is set with cf_copy_in or one of the cf_init_* generators in place of
- * the reference's init functor (src/ProblemManager.hpp:186-263).
+ * (src/VelocityCorrector.hpp:297-339: Jacobi-PCG for "Reference", a
+ * multigrid V-cycle standing in for HYPRE's PCG + PFMG, or plain CG; see
+ * `preconditioner` below).  The initial state is set with cf_copy_in,
+ * cf_copy_in_device or one of the cf_init_* generators in place of the
+ * reference's init functor (src/ProblemManager.hpp:186-263); cf_entity_origin
+ * gives the coordinate that functor is evaluated from.
  */
 typedef struct cf_config {
     int dim;              /* 2 or 3 */
@@ -125,9 +130,22 @@ typedef struct cf_solver cf_solver;
  * would report them (tests/tstMesh.cpp:32-69 pins the 2-D form). */
 cf_status cf_layout_query(const cf_config* cfg, int entity, int ghost_ext[3],
                           int own_lo[3], int own_hi[3]);
-/* Mesh::cellSize (src/Mesh.hpp:541-544) and the effective dt after the
+/* Mesh::cellSize (src/Mesh.hpp:119-122) and the effective dt after the
  * clamp of Solver.hpp:93-106. */
 cf_status cf_geometry_query(const cf_config* cfg, double* dx, double* dt);
+/* The padded device layout every array of a handle shares (the layout of
+ * cf_view): element (i, j, k) of any entity's ghosted local index space is at
+ * offset + i + pitch_x * (j + pitch_y * k), and an array holds `elems`
+ * doubles.  Callers of the *_device entry points allocate arrays of this
+ * size.  Stands in for the Kokkos view layout of ProblemManager::get
+ * (src/ProblemManager.hpp:281-350). */
+cf_status cf_padded_layout(const cf_config* cfg, int64_t* offset, int64_t* pitch_x,
+                           int64_t* pitch_y, int64_t* elems);
+/* Coordinate of local index 0 of an entity: LocalMesh::coordinates(entity,
+ * {0, 0}, loc) as ProblemManager::initialize calls it
+ * (src/ProblemManager.hpp:202-204, 222-223, 243-244).  The init functor is
+ * evaluated at x = origin + cellSize * i for local index i (:210-211). */
+cf_status cf_entity_origin(const cf_config* cfg, int entity, double origin[3]);
 
 /* ---- lifecycle: Solver ctor / createSolver (src/Solver.hpp:69-123, 283-350) */
 cf_status cf_create(const cf_config* cfg, cf_solver** out);
@@ -180,6 +198,15 @@ cf_status cf_copy_out(cf_solver* h, int entity, int version, double* host);
 /* which: 0 lhs (pressure), 1 rhs */
 cf_status cf_copy_pressure_in(cf_solver* h, int which, const double* host);
 cf_status cf_copy_pressure_out(cf_solver* h, int which, double* host);
+/* The same copies from / into a caller's DEVICE array in the padded layout
+ * (cf_padded_layout; every element, ghosts included), by a kernel on the
+ * handle's stream: stream-ordered like every call on the handle.  The
+ * caller's own writes must be complete before cf_copy_in_device (another
+ * stream: synchronise it first), and its reads of the array written by
+ * cf_copy_out_device follow cf_sync.  ProblemManager::get's device views
+ * (src/ProblemManager.hpp:281-350) as copies into caller-owned memory. */
+cf_status cf_copy_in_device(cf_solver* h, int entity, int version, const double* dev);
+cf_status cf_copy_out_device(cf_solver* h, int entity, int version, double* dev);
 cf_status cf_advance(cf_solver* h, int entity);          /* ProblemManager::advance */
 cf_status cf_fill_ghosts(cf_solver* h, int version);     /* ProblemManager::gather */
 cf_status cf_sync(cf_solver* h);
@@ -215,6 +242,18 @@ cf_status cf_adjoint_run(cf_solver* h, int nsteps, int ckpt,
                          const double* dq_T_host, double* du0_host,
                          double* dv0_host, double* dw0_host, double* dq0_host,
                          double dvel[3]);
+/* cf_adjoint_run on device arrays in the padded layout (cf_padded_layout),
+ * stream-ordered on the handle's stream, with no host transfer of any array:
+ * dq_T_dev's non-owned entries are ignored; each gradient array that is not
+ * NULL receives dL/d(initial entity) on the owned entries and 0 elsewhere.
+ * The caller's writes of dq_T_dev must be complete before the call; the
+ * gradients are complete when it returns (dvel is read back to the host).
+ * On any error the handle's Current state and time are those it started
+ * from. */
+cf_status cf_adjoint_run_device(cf_solver* h, int nsteps, int ckpt,
+                                const double* dq_T_dev, double* du0_dev,
+                                double* dv0_dev, double* dw0_dev, double* dq0_dev,
+                                double dvel[3]);
 
 /* ---- output (replaces SiloWriter, src/SiloWriter.hpp:32-422) ------------ */
 /* SiloWriter::writeFile's data (SiloWriter.hpp:141-197): the owned cells of

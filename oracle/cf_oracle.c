@@ -88,7 +88,7 @@ int cfo_create(const cfo_config* cfg, cfo_state** out)
     for (int d = 0; d < 3; ++d)
         s->n[d] = d < s->dim ? cfg->num_cell[d] : 1;
 
-    /* Mesh.hpp:472-486: dx from the x extent; every axis must divide. */
+    /* Mesh.hpp:49-64: dx from the x extent; every axis must divide. */
     s->dx = (cfg->hi[0] - cfg->lo[0]) / cfg->num_cell[0];
     for (int d = 0; d < s->dim; ++d) {
         double extent = cfg->num_cell[d] * s->dx;

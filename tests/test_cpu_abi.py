@@ -151,3 +151,53 @@ def test_create_solver_choices():
     cfg = solver_config(*args, "PCG", "PFMG", mg_nu=3)
     assert cfg["mg_nu"] == 3
     layout(cfg, K.CELL)  # the configuration is valid
+
+
+def test_padded_layout_and_entity_origin():
+    """The padded device layout (cf_padded_layout) holds every entity's
+    ghosted box with the first owned column on a 16-element (128-byte)
+    boundary; cf_entity_origin is the coordinate of local index 0
+    (Cajita LocalMesh: cell l at lo + (l - h + 1/2) dx, a face along its own
+    axis at lo + (l - h) dx)."""
+    from cajitafluids_amd.solver import entity_origin, padded_layout
+    for cfg in (K.dye_tank_2d(128), K.vortex_tank_3d(40), K.periodic_box_3d(24)):
+        off, px, py, elems = padded_layout(cfg)
+        h = cfg["halo"]
+        assert (off + h) % 16 == 0 and px % 16 == 0
+        dx, _ = geometry(cfg)
+        for e in range(cfg["dim"] + 1):
+            ext, lo, hi = layout(cfg, e)
+            assert px >= ext[0] and py >= ext[1]
+            last = off + (ext[0] - 1) + px * ((ext[1] - 1) + py * (ext[2] - 1))
+            assert last < elems
+            o = entity_origin(cfg, e)
+            for d in range(cfg["dim"]):
+                half = 0.0 if e == d + 1 else 0.5
+                want = cfg["lo"][d] + (0 - h + half) * dx
+                assert abs(o[d] - want) <= 1e-15, (e, d, o[d], want)
+
+
+def test_host_entry_shape_checks():
+    """Solver.adjoint_run and set_pressure reject arrays that are not the
+    dense ghosted shape before the library reads the host pointer (an
+    owned-shape array would be read past its end)."""
+    import numpy as np
+    from cajitafluids_amd.solver import Solver
+
+    cfg = K.dye_tank_2d(32)
+
+    class Stub:
+        dim = 2
+
+        def __init__(self):
+            self.cfg = cfg
+
+        def extents(self, e):
+            return layout(cfg, e)[0]
+
+    ext, lo, hi = layout(cfg, K.CELL)
+    owned = np.zeros((1, hi[1] - lo[1], hi[0] - lo[0]))
+    with pytest.raises(ValueError):
+        Solver.adjoint_run(Stub(), 1, 1, owned)
+    with pytest.raises(ValueError):
+        Solver.set_pressure(Stub(), owned)

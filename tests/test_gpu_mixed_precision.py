@@ -60,3 +60,8 @@ def test_refined_vs_fp64_512(cf, fp64_512, mode, itol):
     n = cfg["num_cell"][0]
     kappa = 12.0 / (4.0 * np.sin(np.pi / (2 * n)) ** 2)
     assert err < 2 * cfg["cg_tol"] * kappa
+    # That bound (6.4e-3 here) is loose: the solves are deterministic, and
+    # the errors measured on MI355X are 1.45e-5 (restart, inner 1e-4) and
+    # 9.4e-6 (carry, inner 1e-2).  The test holds them to 2x those values.
+    measured = {K.REFINE_RESTART: 1.45e-5, K.REFINE_CARRY: 9.4e-6}[mode]
+    assert err <= 2.0 * measured, (err, measured)
