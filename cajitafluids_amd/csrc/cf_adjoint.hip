@@ -216,6 +216,9 @@ namespace adjwin
 constexpr int TX = 32, TY = 4, TZ = 2, M = 3;
 constexpr int WX = TX + 2 * M, WY = TY + 2 * M, WZ = TZ + 2 * M;
 constexpr int WN = WX * WY * WZ; // 3040 doubles per array, 97 KB for four
+// the Makefile's ADJ_WIN_LDS: check_lds.py fails the build unless the
+// compiled kernels hold exactly this much LDS
+static_assert(4 * WN * sizeof(double) == 97280, "LDS windows of k_adj_advect_win");
 } // namespace adjwin
 
 struct WindowedScatter

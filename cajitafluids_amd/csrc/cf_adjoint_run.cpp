@@ -144,6 +144,7 @@ cf_status Solver::adjointRun(int nsteps, int ckpt, const double* dq_T,
         return CF_OK;
     };
 
+    TraceRange tr("Adjoint");
     // forward sweep with checkpoints
     copy_arrays(g, fields_of(g, init), _pm->fields(CF_CURRENT), _s);
     cf_status rc = setup(&st);

@@ -159,11 +159,21 @@ namespace TimeIntegrator
 {
 void step(ProblemManager& pm, hipStream_t s, KernelTimer& timer)
 {
+    TraceRange tr("TimeIntegrator::Step");
     const Geo& g = pm.mesh().geo();
-    pm.gather(CF_CURRENT);
-    int slot = timer.begin(s);
-    launch_advect(g, pm.fields(CF_CURRENT), pm.fields(CF_NEXT), s);
-    timer.end(slot, s, TK_ADVECT, 0);
+    {
+        TraceRange t1("TimeIntegrator::Step::Gather");
+        pm.gather(CF_CURRENT);
+    }
+    {
+        // one launch advects every entity (the reference's
+        // ::Advect::Quantity / ::Velocity::FaceI/J/K regions)
+        TraceRange t2("TimeIntegrator::Step::Advect");
+        int slot = timer.begin(s);
+        launch_advect(g, pm.fields(CF_CURRENT), pm.fields(CF_NEXT), s);
+        timer.end(slot, s, TK_ADVECT, 0);
+    }
+    TraceRange t3("CajitaFluids::TimeIntegrator::Advance");
     for (int e = 0; e <= g.dim; ++e)
         pm.advance(e);
 }
@@ -238,7 +248,11 @@ void VelocityCorrector::readState(PcgState& host)
 void VelocityCorrector::buildRHS()
 {
     // VelocityCorrector.hpp:182-212.
-    _pm.gather(CF_CURRENT);
+    TraceRange tr("VelocityCorrector::BuildProblem:buildRHS");
+    {
+        TraceRange t1("VelocityCorrector::BuildProblem:buildRHS:Gather");
+        _pm.gather(CF_CURRENT);
+    }
     launch_divergence(_g, _pm.fields(CF_CURRENT), _rhs.ptr, _s);
 }
 
@@ -252,8 +266,11 @@ void VelocityCorrector::removeMean()
 void VelocityCorrector::applyPressure()
 {
     // VelocityCorrector.hpp:214-264: pressure halo, then the face updates.
-    if (_g.periodic[0] || _g.periodic[1] || _g.periodic[2])
+    TraceRange tr("VelocityCorrector::ApplyPressure");
+    if (_g.periodic[0] || _g.periodic[1] || _g.periodic[2]) {
+        TraceRange t1("VelocityCorrector::ApplyPressure:Gather");
         launch_ghost_fill(_g, CF_CELL, _lhs.ptr, _s);
+    }
     launch_project(_g, _pm.fields(CF_CURRENT), _lhs.ptr, _s);
 }
 
@@ -552,8 +569,17 @@ cf_status VelocityCorrector::correctVelocity(cf_solve_stats* st)
 {
     // VelocityCorrector.hpp:266-282; a solve that does not converge aborts
     // the correction before the projection (the reference's solver throws).
-    buildRHS();
-    cf_status rc = solve(_cfg.precision, _rhs.ptr, _lhs.ptr, st);
+    TraceRange tr("VelocityCorrector");
+    {
+        TraceRange t1("VelocityCorrector::BuildProblem");
+        buildRHS();
+    }
+    cf_status rc;
+    {
+        // the reference's region name, spelling included
+        TraceRange t2("VelocityCorrector::PresureSolve");
+        rc = solve(_cfg.precision, _rhs.ptr, _lhs.ptr, st);
+    }
     if (rc != CF_OK)
         return rc;
     applyPressure();
@@ -599,6 +625,9 @@ void Solver::timeIntegrate() { TimeIntegrator::step(*_pm, _s, _timer); }
 
 void Solver::addInputs()
 {
+    // one launch for every entity (the reference's Solve::AddInputs::Cell /
+    // ::FaceI / ::FaceJ regions)
+    TraceRange tr("Solve::AddInputs");
     launch_inputs(_mesh->geo(), _pm->fields(CF_CURRENT), _s);
 }
 
@@ -621,6 +650,7 @@ cf_status Solver::step(cf_solve_stats* st)
 
 cf_status Solver::run(int nsteps, cf_solve_stats* last)
 {
+    TraceRange tr("Solve");
     for (int n = 0; n < nsteps; ++n) {
         cf_status rc = step(last);
         if (rc != CF_OK)

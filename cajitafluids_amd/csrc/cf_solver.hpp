@@ -6,6 +6,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <array>
 #include <memory>
@@ -31,6 +32,19 @@ struct CfError : std::runtime_error
 
 void hip_check(hipError_t e, const char* what);
 #define CF_HIP(call) ::cfmi::hip_check((call), #call)
+
+// A named range on the host timeline (roctx; rocprofv3 --marker-trace shows
+// the kernels dispatched inside it).  Ranges carry the names of the
+// reference's Kokkos profiling regions around the same phases
+// (Solver.hpp:154-241, TimeIntegrator.hpp:125-163,
+// VelocityCorrector.hpp:184-275).
+struct TraceRange
+{
+    explicit TraceRange(const char* name) { roctxRangePushA(name); }
+    ~TraceRange() { roctxRangePop(); }
+    TraceRange(const TraceRange&) = delete;
+    TraceRange& operator=(const TraceRange&) = delete;
+};
 
 // One device allocation in the handle's padded layout.
 template <class T>

@@ -12,7 +12,9 @@ import numpy as np
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(REPO, "oracle")
-ORACLE_SO = os.path.join(ORACLE_DIR, "libcf_oracle.so")
+# CF_ORACLE_LIBRARY names another build of the same oracle (the sanitizer
+# build of oracle/Makefile's `asan` target)
+ORACLE_SO = os.environ.get("CF_ORACLE_LIBRARY") or os.path.join(ORACLE_DIR, "libcf_oracle.so")
 
 CELL, FACE_X, FACE_Y, FACE_Z = 0, 1, 2, 3
 CURRENT, NEXT = 0, 1

@@ -201,3 +201,23 @@ def test_host_entry_shape_checks():
         Solver.adjoint_run(Stub(), 1, 1, owned)
     with pytest.raises(ValueError):
         Solver.set_pressure(Stub(), owned)
+
+
+def test_trace_ranges_carry_the_reference_region_names():
+    """The host phases are bracketed by roctx ranges (rocprofv3
+    --marker-trace) named after the reference's Kokkos profiling regions
+    (Solver.hpp:154, TimeIntegrator.hpp:125-163, VelocityCorrector.hpp:
+    184-275): the library imports the roctx range calls and holds the names."""
+    import subprocess
+    so = _lib.LIB_PATH
+    syms = subprocess.run(["nm", "-D", so], capture_output=True, text=True).stdout
+    assert "roctxRangePushA" in syms and "roctxRangePop" in syms
+    blob = open(so, "rb").read()
+    for name in ("Solve", "TimeIntegrator::Step", "TimeIntegrator::Step::Gather",
+                 "TimeIntegrator::Step::Advect", "CajitaFluids::TimeIntegrator::Advance",
+                 "VelocityCorrector", "VelocityCorrector::BuildProblem",
+                 "VelocityCorrector::BuildProblem:buildRHS",
+                 "VelocityCorrector::BuildProblem:buildRHS:Gather",
+                 "VelocityCorrector::PresureSolve", "VelocityCorrector::ApplyPressure",
+                 "VelocityCorrector::ApplyPressure:Gather"):
+        assert name.encode() + b"\0" in blob, name
