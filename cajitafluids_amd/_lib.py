@@ -155,6 +155,9 @@ SIGNATURES = {
     "cf_adjoint_run": (C.c_int, [_P, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
     "cf_adjoint_run_device": (C.c_int, [_P, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                         C.c_void_p, C.c_void_p, C.c_void_p, _dp]),
+    "cf_pressure_solve_noalloc": (C.c_int, [C.POINTER(CfConfig), C.c_void_p, C.c_void_p,
+                                            C.c_void_p, _SST]),
+    "cf_noalloc_limits": (C.c_int, [C.POINTER(C.c_size_t), _i64p]),
     "cf_cell_output": (C.c_int, [_P, _dp, _dp]),
     "cf_write_output": (C.c_int, [_P, C.c_char_p, C.c_int]),
     "cf_write_vti": (C.c_int, [C.c_char_p, C.c_int, _ip, _dp, C.c_double, C.c_int,

@@ -528,6 +528,33 @@ cf_status cf_write_output(cf_solver* h, const char* dir, int step)
 }
 
 // No handle: errors are reported through cf_last_error(NULL).
+cf_status cf_pressure_solve_noalloc(const cf_config* cfg, const double* rhs_dev,
+                                    double* lhs_dev, void* stream, cf_solve_stats* st)
+{
+    try {
+        if (!cfg)
+            throw CfError(CF_EINVAL, "null config");
+        return cfmi::pressure_solve_noalloc(*cfg, rhs_dev, lhs_dev,
+                                            static_cast<hipStream_t>(stream), st);
+    } catch (const CfError& e) {
+        g_create_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        return CF_EDEVICE;
+    }
+}
+
+cf_status cf_noalloc_limits(size_t* scratch_bytes, int64_t* max_elems)
+{
+    if (scratch_bytes)
+        *scratch_bytes = cfmi::noalloc_scratch_bytes();
+    if (max_elems)
+        *max_elems = cfmi::noalloc_max_elems();
+    return CF_OK;
+}
+
+// No handle: errors are reported through cf_last_error(NULL).
 cf_status cf_write_vti(const char* path, int dim, const int n[3], const double origin[3],
                        double dx, int cycle, double time, double dt, const double* q,
                        const double* vel)

@@ -266,6 +266,14 @@ void write_vti(const std::string& path, int dim, const int n[3], const double or
                int cycle, double time, double dt, const double* q, const double* vel);
 void write_output(Solver& s, const std::string& dir, int step);
 
+// The no-allocation pressure solve (cf_noalloc.hip): FP64 Jacobi or
+// unpreconditioned PCG on caller device arrays, every work array in one
+// static device array per GPU.
+cf_status pressure_solve_noalloc(const cf_config& cfg, const double* b, double* x,
+                                 hipStream_t s, cf_solve_stats* st);
+size_t noalloc_scratch_bytes();
+int64_t noalloc_max_elems();
+
 // TimeIntegrator::step (src/TimeIntegrator.hpp:120-177): gather, advect all
 // entities into Next in one launch, swap.
 namespace TimeIntegrator

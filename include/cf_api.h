@@ -255,6 +255,29 @@ cf_status cf_adjoint_run_device(cf_solver* h, int nsteps, int ckpt,
                                 double* dv0_dev, double* dw0_dev, double* dq0_dev,
                                 double dvel[3]);
 
+/* ---- no-allocation pressure solve (no counterpart in the reference) ----- */
+/* For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc):
+ * cf_pressure_solve_ext's FP64 solve (Jacobi-PCG, or plain CG with
+ * preconditioner CF_PRECOND_NONE) of A x = b for cfg, with no handle.  rhs_dev
+ * and lhs_dev are the caller's device arrays in the padded layout
+ * (cf_padded_layout); lhs is zeroed first.  Every work array (r, two
+ * directions, the reductions' partials and counters, the PCG scalars) lives
+ * in one static device array of CF_NOALLOC_SCRATCH_BYTES per GPU, held until
+ * the process ends; nothing is allocated.  Grids whose padded layout holds
+ * more than the limit of cf_noalloc_limits (4,183,360 elements: with 3 ghost
+ * layers, 3-D grids up to 147^3 and 2-D grids up to 2028^2) are rejected with CF_EINVAL before any launch;
+ * FP32_REFINED and the multigrid preconditioner give CF_ENOTIMPL.  stream: a
+ * hipStream_t, or NULL for the null stream; the call returns after the solve
+ * (it polls the device for convergence).  Every call on a GPU shares that
+ * GPU's scratch: calls that overlap on one GPU race.  Errors are reported
+ * through cf_last_error(NULL). */
+#define CF_NOALLOC_SCRATCH_BYTES (96u << 20)
+cf_status cf_pressure_solve_noalloc(const cf_config* cfg, const double* rhs_dev,
+                                    double* lhs_dev, void* stream, cf_solve_stats* st);
+/* The scratch size and the largest padded-layout element count it serves. */
+cf_status cf_noalloc_limits(size_t* scratch_bytes, int64_t* max_elems);
+
 /* ---- output (replaces SiloWriter, src/SiloWriter.hpp:32-422) ------------ */
 /* SiloWriter::writeFile's data (SiloWriter.hpp:141-197): the owned cells of
  * q (Current) and the cell-centre velocity from linear interpolation of each
