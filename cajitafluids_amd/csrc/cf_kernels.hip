@@ -88,6 +88,40 @@ void launch_advect(const Geo& g, Fields cur, Fields next, hipStream_t s)
 }
 
 // ---- inputs: Solver::_addInputs (src/Solver.hpp:181-263) -----------------
+// The rule at one owned entity of kind e, local index l: the source
+// (InflowSource.hpp:33-78), the body force (BodyForce.hpp:44-60), then the
+// wall (BoundaryConditions.hpp:102-129), in the reference's order.
+template <int DIM>
+__device__ __forceinline__ void input_rule(const Geo& g, int e, const int (&l)[3],
+                                           double* __restrict__ a)
+{
+    const int64_t c = g.idx(l[0], l[1], l[2]);
+    double x[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+        x[d] = coord(g, e, d, l[d]);
+    double val = a[c];
+    const bool in = in_source(g, x);
+    if (e == 0) {
+        if (in && val < g.Q)
+            val = g.Q;
+    } else {
+        const int d = e - 1;
+        const double U = g.U[d];
+        if (in && fabs(val) < fabs(U))
+            val = U;
+        val = val + g.force[d] * g.dt;
+        if (!g.periodic[d]) {
+            const int gl = l[d] - g.h;
+            if (gl <= 0 && g.solid[d][0])
+                val = 0;
+            if (gl > g.n[d] - 1 && g.solid[d][1])
+                val = 0;
+        }
+    }
+    a[c] = val;
+}
+
 template <int DIM>
 __global__ __launch_bounds__(256) void k_inputs(Geo g, Fields cur)
 {
@@ -95,45 +129,131 @@ __global__ __launch_bounds__(256) void k_inputs(Geo g, Fields cur)
     const int j = g.h + blockIdx.y * blockDim.y + threadIdx.y;
     const int k = g.own_lo(2) + blockIdx.z;
     const int l[3] = {i, j, k};
-    const int64_t c = g.idx(i, j, k);
 #pragma unroll
     for (int e = 0; e <= DIM; ++e) {
         bool own = true;
 #pragma unroll
         for (int d = 0; d < DIM; ++d)
             own = own && l[d] < g.own_hi[e][d];
-        if (!own)
-            continue;
-        double x[3] = {0, 0, 0};
-#pragma unroll
-        for (int d = 0; d < DIM; ++d)
-            x[d] = coord(g, e, d, l[d]);
-        double* a = cur.e[e];
-        double val = a[c];
-        const bool in = in_source(g, x);
-        if (e == 0) {
-            if (in && val < g.Q)
-                val = g.Q;
-        } else {
-            const int d = e - 1;
-            const double U = g.U[d];
-            if (in && fabs(val) < fabs(U))
-                val = U;
-            val = val + g.force[d] * g.dt;
-            if (!g.periodic[d]) {
-                const int gl = l[d] - g.h;
-                if (gl <= 0 && g.solid[d][0])
-                    val = 0;
-                if (gl > g.n[d] - 1 && g.solid[d][1])
-                    val = 0;
-            }
-        }
-        a[c] = val;
+        if (own)
+            input_rule<DIM>(g, e, l, cur.e[e]);
     }
+}
+
+// With no body force the rule changes an entity only inside the inflow
+// region or on a SOLID wall (elsewhere it adds 0 * dt, which leaves every
+// value but a -0.0 as it is), so the inputs visit only boxes of local
+// indices that cover those: per entity, the inflow region's bounding box
+// widened by one index, and each SOLID wall plane of a face entity.  The
+// rule is idempotent, so a cell in two boxes gets the same value twice.
+struct InputBoxes
+{
+    int n;
+    int e[kMaxInputBoxes];
+    int lo[kMaxInputBoxes][3], hi[kMaxInputBoxes][3];
+    int64_t start[kMaxInputBoxes + 1]; // prefix sums of the box volumes
+};
+
+template <int DIM>
+__global__ __launch_bounds__(256) void k_inputs_boxes(Geo g, Fields cur, InputBoxes bx)
+{
+    const int64_t total = bx.start[bx.n];
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        int b = 0;
+        while (t >= bx.start[b + 1])
+            ++b;
+        const int64_t u = t - bx.start[b];
+        const int nx = bx.hi[b][0] - bx.lo[b][0], ny = bx.hi[b][1] - bx.lo[b][1];
+        const int l[3] = {bx.lo[b][0] + (int)(u % nx), bx.lo[b][1] + (int)((u / nx) % ny),
+                          bx.lo[b][2] + (int)(u / ((int64_t)nx * ny))};
+        input_rule<DIM>(g, bx.e[b], l, cur.e[bx.e[b]]);
+    }
+}
+
+static InputBoxes input_boxes(const Geo& g)
+{
+    InputBoxes bx;
+    bx.n = 0;
+    auto add = [&](int e, int lo[3], int hi[3]) {
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = max(lo[d], g.own_lo(d));
+            hi[d] = min(hi[d], g.own_hi[e][d]);
+            if (hi[d] <= lo[d])
+                return;
+        }
+        bx.e[bx.n] = e;
+        for (int d = 0; d < 3; ++d) {
+            bx.lo[bx.n][d] = lo[d];
+            bx.hi[bx.n][d] = hi[d];
+        }
+        ++bx.n;
+    };
+    if (g.inflow_shape != CF_INFLOW_NONE) {
+        for (int e = 0; e <= g.dim; ++e) {
+            int lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+            for (int d = 0; d < g.dim; ++d) {
+                double a, b;
+                if (g.inflow_shape == CF_INFLOW_BOX) {
+                    a = g.in_lo[d];
+                    b = g.in_hi[d];
+                } else {
+                    const double r = sqrt(g.in_r2);
+                    a = g.in_c[d] - r;
+                    b = g.in_c[d] + r;
+                }
+                // coordinate(l) = x0 + l dx: widened by one index each way
+                lo[d] = (int)floor((a - g.x0[e][d]) / g.dx) - 1;
+                hi[d] = (int)ceil((b - g.x0[e][d]) / g.dx) + 2;
+            }
+            add(e, lo, hi);
+        }
+    }
+    for (int e = 1; e <= g.dim; ++e) {
+        const int d = e - 1;
+        if (g.periodic[d])
+            continue;
+        for (int side = 0; side < 2; ++side) {
+            if (!g.solid[d][side])
+                continue;
+            int lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+            for (int m = 0; m < g.dim; ++m) {
+                lo[m] = g.own_lo(m);
+                hi[m] = g.own_hi[e][m];
+            }
+            lo[d] = side == 0 ? g.h : g.h + g.n[d];
+            hi[d] = lo[d] + 1;
+            add(e, lo, hi);
+        }
+    }
+    bx.start[0] = 0;
+    for (int b = 0; b < bx.n; ++b)
+        bx.start[b + 1] = bx.start[b] + (int64_t)(bx.hi[b][0] - bx.lo[b][0]) *
+                                             (bx.hi[b][1] - bx.lo[b][1]) *
+                                             (bx.hi[b][2] - bx.lo[b][2]);
+    return bx;
+}
+
+bool inputs_are_sparse(const Geo& g)
+{
+    return g.force[0] == 0.0 && g.force[1] == 0.0 && g.force[2] == 0.0;
 }
 
 void launch_inputs(const Geo& g, Fields cur, hipStream_t s)
 {
+    if (inputs_are_sparse(g)) {
+        const InputBoxes bx = input_boxes(g);
+        const int64_t total = bx.start[bx.n];
+        if (total == 0)
+            return;
+        int64_t blocks = (total + 255) / 256;
+        const unsigned nb = (unsigned)(blocks < 2048 ? blocks : 2048);
+        if (g.dim == 2)
+            hipLaunchKernelGGL(k_inputs_boxes<2>, dim3(nb), dim3(256), 0, s, g, cur, bx);
+        else
+            hipLaunchKernelGGL(k_inputs_boxes<3>, dim3(nb), dim3(256), 0, s, g, cur, bx);
+        return;
+    }
     dim3 blk(64, 4, 1);
     if (g.dim == 2)
         hipLaunchKernelGGL(k_inputs<2>, union_grid(g, blk), blk, 0, s, g, cur);

@@ -44,7 +44,12 @@ struct Fields
 };
 
 void launch_advect(const Geo& g, Fields cur, Fields next, hipStream_t s);
+// Solver::_addInputs.  With no body force only the inflow region's
+// bounding box and the SOLID wall planes of the face entities are visited
+// (at most kMaxInputBoxes index boxes); with one, every owned entity.
+constexpr int kMaxInputBoxes = 12;
 void launch_inputs(const Geo& g, Fields cur, hipStream_t s);
+bool inputs_are_sparse(const Geo& g);
 void launch_ghost_fill(const Geo& g, int entity, double* a, hipStream_t s);
 void launch_divergence(const Geo& g, Fields cur, double* rhs, hipStream_t s);
 void launch_rhs_sum(const Geo& g, const double* rhs, double* out_sum,

@@ -92,7 +92,7 @@ def test_exact_extent_reproduces_the_box_corner():
 @pytest.mark.gpu
 def test_adapter_run_matches_python_mirror(tmp_path):
     from cajitafluids_amd.solver import Solver, entity_origin, geometry, layout
-    n, nsteps = 32, 3
+    n, nsteps = 128, 3  # dx / (3 + sqrt(0.5 dx)) < 0.005: the dt clamp applies
     out = tmp_path / "demo.bin"
     print(f"binding_demo {n} {nsteps}", flush=True)
     r = subprocess.run([BIN, str(n), str(nsteps), str(out)], capture_output=True, text=True,
