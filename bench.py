@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--mg-coarse", type=int, default=None,
                     help="multigrid leg: sweeps on the coarsest level")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-dye", action="store_true",
+                    help="skip the 2-D 128^2 dye-tank leg (profiling runs)")
     ap.add_argument("--cpu-iters", type=int, default=400)
     ap.add_argument("--timing-sample", type=int, default=8,
                     help="time the PCG kernels with HIP events in one "
@@ -336,7 +338,8 @@ def main():
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, n, total_iters / args.steps, args.cpu_iters)
-    out["dye_tank_128"] = dye_tank_leg(not args.no_cpu)
+    if not args.no_dye:
+        out["dye_tank_128"] = dye_tank_leg(not args.no_cpu)
 
     print(json.dumps(out))
 

@@ -194,7 +194,7 @@ __global__ __launch_bounds__(256) void k_mg_smooth3(Geo g, Geo gw, Tile3Grid tg,
     using namespace tile3;
     if (st->done)
         return;
-    __shared__ double pl[2][TY + 2][tx<1>() + 2];
+    __shared__ double pl[2][TY + 2][prow<1>()];
     MgSweepEpi<RZ, MODE == 1> epi{gw, b, xo};
     if constexpr (MODE == 1)
         tile3_march<double, 1, 1>(gw, tg, b, nullptr, true, 0.0, pl, epi);
@@ -290,7 +290,7 @@ __global__ __launch_bounds__(256) void k_mg_resrestrict3(Geo g, Tile3Grid tg, Mg
     using namespace tile3;
     if (st->done)
         return;
-    __shared__ double pl[2][TY + 2][tx<1>() + 2];
+    __shared__ double pl[2][TY + 2][prow<1>()];
     MgRestrictEpi epi{g, b, bc, pm};
     tile3_march<double, 0, 1>(g, tg, x, nullptr, false, 0.0, pl, epi);
 }

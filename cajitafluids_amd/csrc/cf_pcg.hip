@@ -323,7 +323,7 @@ __global__ __launch_bounds__(256) void k_pcg_dir_spmv3(
     using namespace tile3;
     if (st->done)
         return;
-    __shared__ T pl[2][TY + 2][tx<CPL>() + 2];
+    __shared__ __align__(16) T pl[2][TY + 2][prow<CPL>()];
     const bool first = st->iter == 0;
     const T beta = (T)st->beta;
     DirEpiT<T, CPL> epi{g, p_new};
@@ -341,7 +341,7 @@ __global__ __launch_bounds__(256) void k_pcg_update3(
     using namespace tile3;
     if (st->done)
         return;
-    __shared__ T pl[2][TY + 2][tx<CPL>() + 2];
+    __shared__ __align__(16) T pl[2][TY + 2][prow<CPL>()];
     UpdEpiT<T, CPL> epi{g, x, r, (T)st->alpha};
     // downwards: the direction kernel marches up, so each kernel starts on
     // the planes the other touched last (Infinity Cache hits; FP64 256^3:

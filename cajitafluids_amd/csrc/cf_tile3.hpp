@@ -55,6 +55,21 @@ constexpr int tx()
 {
     return LX * CPL; // tile cells along x
 }
+// LDS plane layout: a row holds the tile's cells from column pcol() on,
+// with the x-halo cells just left and right of them.  With two cells per
+// lane the first cell sits at column 2, so each lane's pair, and the pairs
+// beside and above / below it, are 8-byte aligned: every LDS access of the
+// stencil is one conflict-free 8-byte access per lane.
+template <int CPL>
+constexpr int pcol()
+{
+    return CPL;
+}
+template <int CPL>
+constexpr int prow()
+{
+    return tx<CPL>() + 2 * pcol<CPL>();
+}
 } // namespace tile3
 
 struct Tile3Grid
@@ -314,7 +329,7 @@ __device__ __forceinline__ Tile3Ctx<CPL> tile3_ctx(const Geo& g, const Tile3Grid
         const int gi = c.i0 + l2 * CPL;
         c.hkind = 1;
         c.hrow = hh < LX ? 0 : TY + 1;
-        c.hcol = l2 * CPL + 1;
+        c.hcol = l2 * CPL + pcol<CPL>();
         c.colb[RY] = col(gi, gj);
         c.sgi[RY] = wrap_or_ghost(g, 0, gi);
         c.sgj[RY] = wrap_or_ghost(g, 1, gj);
@@ -333,7 +348,7 @@ __device__ __forceinline__ Tile3Ctx<CPL> tile3_ctx(const Geo& g, const Tile3Grid
         const int gj = c.j0 + r;
         c.hkind = 2;
         c.hrow = r + 1;
-        c.hcol = left ? 0 : tx<CPL>() + 1;
+        c.hcol = left ? pcol<CPL>() - 1 : tx<CPL>() + pcol<CPL>();
         c.colb[RY] = col(gi, gj);
         c.sgi[RY] = wrap_or_ghost(g, 0, gi);
         c.sgj[RY] = wrap_or_ghost(g, 1, gj);
@@ -470,27 +485,28 @@ template <class T, int DIR, int CPL>
 __device__ __forceinline__ void tile3_finish(const Geo& g, const Tile3Ctx<CPL>& c,
                                              const Plane3Meta& m,
                                              const Plane3<T, DIR, CPL>& P, bool first,
-                                             T beta, T (*pl)[tile3::tx<CPL>() + 2],
+                                             T beta, T (*pl)[tile3::prow<CPL>()],
                                              T (&own)[tile3::RY][CPL])
 {
     using namespace tile3;
 #pragma unroll
     for (int q = 0; q < RY; ++q) {
 #pragma unroll
-        for (int j = 0; j < CPL; ++j) {
+        for (int j = 0; j < CPL; ++j)
             own[q][j] = tile3_value<T, DIR, CPL>(g, c, m, P, q, j, first, beta);
-            pl[c.w * RY + q + 1][c.lane * CPL + j + 1] = own[q][j];
-        }
+        // one aligned access per lane (a pair with two cells per lane)
+        vstore<T, CPL>(&pl[c.w * RY + q + 1][c.lane * CPL + pcol<CPL>()], own[q]);
     }
     if constexpr (CPL == 1) {
         if (c.hkind != 0)
             pl[c.hrow][c.hcol] = tile3_value<T, DIR, CPL>(g, c, m, P, RY, 0, first, beta);
     } else {
         if (c.hkind == 1) {
+            T hv[CPL];
 #pragma unroll
             for (int j = 0; j < CPL; ++j)
-                pl[c.hrow][c.hcol + j] =
-                    tile3_value<T, DIR, CPL>(g, c, m, P, RY, j, first, beta);
+                hv[j] = tile3_value<T, DIR, CPL>(g, c, m, P, RY, j, first, beta);
+            vstore<T, CPL>(&pl[c.hrow][c.hcol], hv);
         } else if (c.hkind == 2) {
             pl[c.hrow][c.hcol] = tile3_value<T, DIR, CPL>(g, c, m, P, RY, 0, first, beta);
         }
@@ -501,22 +517,60 @@ __device__ __forceinline__ void tile3_finish(const Geo& g, const Tile3Ctx<CPL>& 
 // pm / pp, x neighbours from the lane's group or the LDS plane, y neighbours
 // from the LDS plane; reference stencil order {0}, -x, +x, -y, +y, -z, +z
 // with zero couplings across walls.
+// The x and y neighbours of a lane's cells in own row q, from the LDS
+// plane (the x neighbours inside a lane's pair come from registers).  With
+// two cells per lane: the pairs left, right, above and below the lane's
+// pair, four aligned 8-byte reads.
+template <class T, int CPL>
+struct RowNb
+{
+    T xm[CPL], xp[CPL], ym[CPL], yp[CPL];
+};
+
+template <class T, int CPL>
+__device__ __forceinline__ RowNb<T, CPL> tile3_neighbours(const Tile3Ctx<CPL>& c,
+                                                          const T (*pl)[tile3::prow<CPL>()],
+                                                          int q, const T (&pc)[CPL])
+{
+    using namespace tile3;
+    RowNb<T, CPL> nb;
+    const int jj = c.w * RY + q;
+    const int c0 = c.lane * CPL + pcol<CPL>(); // LDS column of the lane's first cell
+    if constexpr (CPL == 1) {
+        nb.xm[0] = pl[jj + 1][c0 - 1];
+        nb.xp[0] = pl[jj + 1][c0 + 1];
+        nb.ym[0] = pl[jj][c0];
+        nb.yp[0] = pl[jj + 2][c0];
+    } else {
+        T L[2], R[2], U[2], D[2];
+        vload<T, 2>(&pl[jj + 1][c0 - 2], L);
+        vload<T, 2>(&pl[jj + 1][c0 + 2], R);
+        vload<T, 2>(&pl[jj][c0], U);
+        vload<T, 2>(&pl[jj + 2][c0], D);
+        nb.xm[0] = L[1];
+        nb.xp[0] = pc[1];
+        nb.xm[1] = pc[0];
+        nb.xp[1] = R[0];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            nb.ym[j] = U[j];
+            nb.yp[j] = D[j];
+        }
+    }
+    return nb;
+}
+
 template <class T, int CPL>
 __device__ __forceinline__ T tile3_apply(const Geo& g, const Tile3Ctx<CPL>& c,
-                                         const Plane3Meta& m,
-                                         const T (*pl)[tile3::tx<CPL>() + 2], int q,
+                                         const Plane3Meta& m, const RowNb<T, CPL>& nb, int q,
                                          int j, const T (&pc)[CPL], T pm, T pp)
 {
     const T s = offdiag<T>(g);
-    const int jj = c.w * tile3::RY + q;
-    const int ln = c.lane * CPL + j; // LDS column of the cell is ln + 1
-    const T xm = j > 0 ? pc[j > 0 ? j - 1 : 0] : pl[jj + 1][ln];
-    const T xp = j < CPL - 1 ? pc[j < CPL - 1 ? j + 1 : 0] : pl[jj + 1][ln + 2];
     T y = diag_pick<T>(g, c.wy[q] + m.wz, c.wx[j]) * pc[j];
-    y = fma_t<T>(c.xwall_m[j] ? (T)0 : s, xm, y);
-    y = fma_t<T>(c.xwall_p[j] ? (T)0 : s, xp, y);
-    y = fma_t<T>(c.ywall_m[q] ? (T)0 : s, pl[jj][ln + 1], y);
-    y = fma_t<T>(c.ywall_p[q] ? (T)0 : s, pl[jj + 2][ln + 1], y);
+    y = fma_t<T>(c.xwall_m[j] ? (T)0 : s, nb.xm[j], y);
+    y = fma_t<T>(c.xwall_p[j] ? (T)0 : s, nb.xp[j], y);
+    y = fma_t<T>(c.ywall_m[q] ? (T)0 : s, nb.ym[j], y);
+    y = fma_t<T>(c.ywall_p[q] ? (T)0 : s, nb.yp[j], y);
     y = fma_t<T>(m.zwall_m ? (T)0 : s, pm, y);
     y = fma_t<T>(m.zwall_p ? (T)0 : s, pp, y);
     return y;
@@ -548,7 +602,7 @@ template <class T, int DIR, int CPL, bool PX = false, bool REV = false, class Ep
 __device__ __forceinline__ void tile3_march(
     const Geo& g, const Tile3Grid& tg, const T* __restrict__ va,
     const T* __restrict__ vb, bool first, T beta,
-    T (*pl)[tile3::TY + 2][tile3::tx<CPL>() + 2], Epi&& epi,
+    T (*pl)[tile3::TY + 2][tile3::prow<CPL>()], Epi&& epi,
     const Tile3Parent par = Tile3Parent{nullptr, 0, 0, 0, 0.0})
 {
     using namespace tile3;
@@ -603,10 +657,11 @@ __device__ __forceinline__ void tile3_march(
         for (int q = 0; q < RY; ++q) {
             // a group with no valid cell has nothing to compute or store
             if (c.valid[q][0]) {
+                const RowNb<T, CPL> nb = tile3_neighbours<T, CPL>(c, pl[S], q, pc[q]);
                 T y[CPL];
 #pragma unroll
                 for (int j = 0; j < CPL; ++j)
-                    y[j] = tile3_apply<T, CPL>(g, c, mc, pl[S], q, j, pc[q],
+                    y[j] = tile3_apply<T, CPL>(g, c, mc, nb, q, j, pc[q],
                                                REV ? pp[q][j] : pm[q][j],
                                                REV ? pm[q][j] : pp[q][j]);
                 epi.template cells<S>(c, mc, q, pc[q], y);
@@ -640,10 +695,11 @@ __device__ __forceinline__ void tile3_march(
 #pragma unroll
             for (int q = 0; q < RY; ++q) {
                 if (c.valid[q][0]) {
+                    const RowNb<T, CPL> nb = tile3_neighbours<T, CPL>(c, pl[cur], q, pc[q]);
                     T y[CPL];
 #pragma unroll
                     for (int j = 0; j < CPL; ++j)
-                        y[j] = tile3_apply<T, CPL>(g, c, mc, pl[cur], q, j, pc[q],
+                        y[j] = tile3_apply<T, CPL>(g, c, mc, nb, q, j, pc[q],
                                                    REV ? pp[q][j] : pm[q][j],
                                                    REV ? pm[q][j] : pp[q][j]);
                     epi.template cells<0>(c, mc, q, pc[q], y);
