@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--no-dye", action="store_true",
                     help="skip the 2-D 128^2 dye-tank leg (profiling runs)")
     ap.add_argument("--cpu-iters", type=int, default=400)
-    ap.add_argument("--timing-sample", type=int, default=8,
+    ap.add_argument("--timing-sample", type=int, default=16,
                     help="time the PCG kernels with HIP events in one "
                          "iteration out of N inside the timed region (0: off)")
     ap.add_argument("--adjoint", action="store_true",
@@ -73,6 +73,13 @@ def parse():
                     help="only time plain streaming kernels (torch's add: read 2, write "
                          "1; and copy) over 263^3 FP64 arrays, the padded 256^3 size, "
                          "and print their rates")
+    ap.add_argument("--dye-only", action="store_true",
+                    help="only the 2-D 128^2 dye tank (setup + 50 steps) on the GPU "
+                         "(profiling runs; prints its object)")
+    ap.add_argument("--advect-only", action="store_true",
+                    help="only the advection kernel: 4 launches each on the 256^3 vortex "
+                         "tank and the 384^3 periodic box after setup + one step, timed "
+                         "with HIP events (profiling runs; prints the mean ms)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a plain 1 GiB device copy, the reference "
                          "kernel for scaling rocprof FETCH_SIZE / WRITE_SIZE")
@@ -220,6 +227,12 @@ def main():
         return
     if args.stream_probe:
         stream_probe()
+        return
+    if args.dye_only:
+        print(json.dumps({"dye_tank_128": dye_tank_leg(False)}))
+        return
+    if args.advect_only:
+        print(json.dumps({"advect_ms": advect_only()}))
         return
     if args.copy_probe:
         copy_probe()
@@ -370,6 +383,36 @@ def advect_figures(cfg, avg_ms):
                    tflops=round(tf, 2), fp64_peak_frac=round(tf / FP64_PEAK_TFLOPS, 4),
                    hbm_frac=round(out["gbs_unique"] / HBM_PEAK_GBS, 4),
                    bound="gathers: 136 taps per point through L1/L2, three RK3 stages in sequence")
+    return out
+
+
+def advect_only():
+    """Mean advection kernel time (HIP events, 4 launches) on the 256^3 vortex
+    tank and on the 384^3 periodic box (FP64, multigrid solver so that the
+    setup is quick), each after setup and one step."""
+    from cajitafluids_amd import _lib, configs as K
+    from cajitafluids_amd.solver import Solver
+    out = {}
+    tank = K.vortex_tank_3d(256)
+    tank["preconditioner"] = K.PRECOND_MG
+    box = K.periodic_box_3d(384)
+    box.update(precision=K.FP64, preconditioner=K.PRECOND_MG)
+    for name, cfg, init in (
+            ("vortex_tank_256", tank, lambda s: s.init_vortex_tubes(K.VORTEX_TANK_SEED, 1.0)),
+            ("periodic_box_384", box, lambda s: (s.init_vortex_tubes(K.PERIODIC_BOX_VEL_SEED, 1.0),
+                                                 s.init_blobs(K.PERIODIC_BOX_DYE_SEED, 8)))):
+        with Solver(cfg) as s:
+            init(s)
+            s.setup()
+            s.step()
+            s.sync()
+            s.reset_kernel_time()
+            s.set_kernel_timing(True)
+            for _ in range(4):
+                s.time_integrate()
+            s.sync()
+            ms, n = s.kernel_time(_lib.TK_ADVECT)
+            out[name] = round(ms / n, 4)
     return out
 
 

@@ -136,17 +136,16 @@ __global__ __launch_bounds__(256) void k_pcg_init32(
 }
 
 // ---- one PCG iteration --------------------------------------------------------
+// The two halves of an iteration over this block's owned rows (the row
+// partition of for_owned_rows), accumulating the block's double-double
+// partials.
 template <int DIM, class T>
-__global__ __launch_bounds__(256) void k_pcg_dir_spmv(
-    Geo g, const T* __restrict__ r, const T* __restrict__ p_old,
-    T* __restrict__ p_new, PcgState* st, RedBuf rb)
+__device__ __forceinline__ void dir_spmv_rows(const Geo& g, const T* __restrict__ r,
+                                              const T* __restrict__ p_old,
+                                              T* __restrict__ p_new, bool first, T beta,
+                                              double (&hi)[1], double (&lo)[1])
 {
-    if (st->done)
-        return;
-    const bool first = st->iter == 0;
-    const T beta = (T)st->beta;
     const T s = offdiag<T>(g);
-    double hi[1] = {0.0}, lo[1] = {0.0};
     for_owned_rows(g, [&](int64_t base, int j, int k) {
         const RowStencil rs = row_stencil(g, j, k);
         for (int i = threadIdx.x; i < g.n[0]; i += kBlock) {
@@ -177,18 +176,13 @@ __global__ __launch_bounds__(256) void k_pcg_dir_spmv(
             dd_add_prod(hi[0], lo[0], (double)pc, (double)y);
         }
     });
-    grid_reduce<1>(hi, lo, rb, [&](double* sm) { st->alpha = st->rho / sm[0]; });
 }
 
 template <int DIM, class T>
-__global__ __launch_bounds__(256) void k_pcg_update(
-    Geo g, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
-    PcgState* st, RedBuf rb)
+__device__ __forceinline__ void update_rows(const Geo& g, T* __restrict__ x, T* __restrict__ r,
+                                            const T* __restrict__ p, T alpha, double (&hi)[2],
+                                            double (&lo)[2])
 {
-    if (st->done)
-        return;
-    const T alpha = (T)st->alpha;
-    double hi[2] = {0.0, 0.0}, lo[2] = {0.0, 0.0};
     for_owned_rows(g, [&](int64_t base, int j, int k) {
         const RowStencil rs = row_stencil(g, j, k);
         for (int i = threadIdx.x; i < g.n[0]; i += kBlock) {
@@ -203,6 +197,29 @@ __global__ __launch_bounds__(256) void k_pcg_update(
             dd_add_prod(hi[1], lo[1], (double)rn, (double)z);
         }
     });
+}
+
+template <int DIM, class T>
+__global__ __launch_bounds__(256) void k_pcg_dir_spmv(
+    Geo g, const T* __restrict__ r, const T* __restrict__ p_old,
+    T* __restrict__ p_new, PcgState* st, RedBuf rb)
+{
+    if (st->done)
+        return;
+    double hi[1] = {0.0}, lo[1] = {0.0};
+    dir_spmv_rows<DIM, T>(g, r, p_old, p_new, st->iter == 0, (T)st->beta, hi, lo);
+    grid_reduce<1>(hi, lo, rb, [&](double* sm) { st->alpha = st->rho / sm[0]; });
+}
+
+template <int DIM, class T>
+__global__ __launch_bounds__(256) void k_pcg_update(
+    Geo g, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
+    PcgState* st, RedBuf rb)
+{
+    if (st->done)
+        return;
+    double hi[2] = {0.0, 0.0}, lo[2] = {0.0, 0.0};
+    update_rows<DIM, T>(g, x, r, p, (T)st->alpha, hi, lo);
     grid_reduce<2>(hi, lo, rb, [&](double* sm) { finish_update(st, sm[0], sm[1]); });
 }
 
