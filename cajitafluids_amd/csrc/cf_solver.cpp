@@ -2,11 +2,9 @@
 // VelocityCorrector, TimeIntegrator, Solver) driving the HIP kernels.
 #include "cf_solver.hpp"
 
-#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstring>
-#include <iterator>
 
 #include "cf_init.hpp"
 
@@ -218,8 +216,6 @@ VelocityCorrector::VelocityCorrector(ProblemManager& pm, const cf_config& cfg,
 
 VelocityCorrector::~VelocityCorrector()
 {
-    for (auto& g : _graphs)
-        (void)hipGraphExecDestroy(g.exec);
     if (_st)
         (void)hipFree(_st);
     if (_rs)
@@ -278,70 +274,6 @@ void VelocityCorrector::applyPressure()
     launch_project(_g, _pm.fields(CF_CURRENT), _lhs.ptr, _s);
 }
 
-hipGraphExec_t VelocityCorrector::cachedGraph(const void* const (&key)[6],
-                                              const std::function<void()>& body)
-{
-    for (auto& g : _graphs)
-        if (std::equal(std::begin(key), std::end(key), std::begin(g.key)))
-            return g.exec;
-    // kernel launches only: nothing that must not be recorded into a graph
-    hipGraph_t graph = nullptr;
-    CF_HIP(hipStreamBeginCapture(_s, hipStreamCaptureModeThreadLocal));
-    try {
-        body();
-    } catch (...) {
-        if (hipStreamEndCapture(_s, &graph) == hipSuccess && graph)
-            (void)hipGraphDestroy(graph);
-        throw;
-    }
-    CF_HIP(hipStreamEndCapture(_s, &graph));
-    hipGraphExec_t exec = nullptr;
-    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    CF_HIP(e);
-    CachedGraph c;
-    std::copy(std::begin(key), std::end(key), std::begin(c.key));
-    c.exec = exec;
-    _graphs.push_back(c);
-    return exec;
-}
-
-template <class T>
-hipGraphExec_t VelocityCorrector::pcgGraph(T* x, T* r, DevArray<T> (&p)[2], int cur)
-{
-    const Geo* geo = &pcgGeo();
-    const void* key[6] = {x, r, p[0].ptr, p[1].ptr, geo,
-                          reinterpret_cast<const void*>((intptr_t)(cur + 2 * sizeof(T)))};
-    return cachedGraph(key, [&]() {
-        RedBuf rb{_red_partials.ptr, _red_counter.ptr};
-        int c = cur;
-        for (int i = 0; i < kGraphIters; ++i) {
-            launch_pcg_dir_spmv<T>(*geo, r, p[c].ptr, p[c ^ 1].ptr, _st, rb, _s);
-            launch_pcg_update<T>(*geo, x, r, p[c ^ 1].ptr, _st, rb, _s);
-            c ^= 1;
-        }
-    });
-}
-
-hipGraphExec_t VelocityCorrector::mgGraph(double* x, const double* z, int cur)
-{
-    const void* key[6] = {x, z, _p[0].ptr, _p[1].ptr, _mg.get(),
-                          reinterpret_cast<const void*>((intptr_t)(cur + 100))};
-    return cachedGraph(key, [&]() {
-        RedBuf rb{_red_partials.ptr, _red_counter.ptr};
-        int c = cur;
-        const double* zz = z;
-        for (int i = 0; i < kGraphItersMg; ++i) {
-            launch_pcg_dir_spmv<double>(_g_unit, zz, _p[c].ptr, _p[c ^ 1].ptr, _st, rb, _s);
-            launch_pcg_update<double>(_g, x, _r.ptr, _p[c ^ 1].ptr, _st, rb, _s);
-            zz = _mg->vcycle(_r.ptr, _st, rb);
-            c ^= 1;
-        }
-        if (zz != z)
-            throw CfError(CF_EDEVICE, "V-cycle output moved between iterations");
-    });
-}
-
 // Queue batches of iterations without waiting; poll the device flag once
 // per batch.  The first batch is sized from a prediction (the previous
 // solve's, or pass's, count) so that few no-op launches follow the stop.
@@ -360,24 +292,15 @@ int VelocityCorrector::runIterations(T* x, T* r, DevArray<T> (&p)[2], int& cur,
     int launched = 0;
     int batch = predict > 4 ? predict - 2 : 16;
     for (;;) {
-        if (!_timer.enabled) {
-            // whole graphs: cur is unchanged after an even count
-            const int ng = (batch + kGraphIters - 1) / kGraphIters;
-            hipGraphExec_t ex = pcgGraph<T>(x, r, p, cur);
-            for (int gi = 0; gi < ng; ++gi)
-                CF_HIP(hipGraphLaunch(ex, _s));
-            batch = ng * kGraphIters;
-        } else {
-            for (int c = 0; c < batch; ++c) {
-                const int it = it0 + launched + c + 1;
-                int t0 = _timer.begin_iter(_s, it);
-                launch_pcg_dir_spmv<T>(pcgGeo(), r, p[cur].ptr, p[cur ^ 1].ptr, _st, rb, _s);
-                _timer.end(t0, _s, tk0 + 0, it);
-                int t2 = _timer.begin_iter(_s, it);
-                launch_pcg_update<T>(pcgGeo(), x, r, p[cur ^ 1].ptr, _st, rb, _s);
-                _timer.end(t2, _s, tk0 + 1, it);
-                cur ^= 1;
-            }
+        for (int c = 0; c < batch; ++c) {
+            const int it = it0 + launched + c + 1;
+            int t0 = _timer.begin_iter(_s, it);
+            launch_pcg_dir_spmv<T>(pcgGeo(), r, p[cur].ptr, p[cur ^ 1].ptr, _st, rb, _s);
+            _timer.end(t0, _s, tk0 + 0, it);
+            int t2 = _timer.begin_iter(_s, it);
+            launch_pcg_update<T>(pcgGeo(), x, r, p[cur ^ 1].ptr, _st, rb, _s);
+            _timer.end(t2, _s, tk0 + 1, it);
+            cur ^= 1;
         }
         launched += batch;
         readState(hs);
@@ -601,28 +524,19 @@ cf_status VelocityCorrector::solveMg(const double* b, double* x, cf_solve_stats*
     int batch = _predict_mg > 3 ? _predict_mg - 2 : 8;
     const int it0 = hs.iter;
     while (!hs.done) {
-        if (!_timer.enabled) {
-            const int ng = (batch + kGraphItersMg - 1) / kGraphItersMg;
-            hipGraphExec_t ex = mgGraph(x, z, cur);
-            for (int gi = 0; gi < ng; ++gi)
-                CF_HIP(hipGraphLaunch(ex, _s));
-            batch = ng * kGraphItersMg;
-        } else {
-            for (int c = 0; c < batch; ++c) {
-                const int it = it0 + launched + c + 1;
-                int t0 = _timer.begin_iter(_s, it);
-                launch_pcg_dir_spmv<double>(_g_unit, z, _p[cur].ptr, _p[cur ^ 1].ptr, _st, rb,
-                                            _s);
-                _timer.end(t0, _s, TK_DIR_SPMV, it);
-                int t1 = _timer.begin_iter(_s, it);
-                launch_pcg_update<double>(_g, x, _r.ptr, _p[cur ^ 1].ptr, _st, rb, _s);
-                _timer.end(t1, _s, TK_UPDATE, it);
-                // the V-cycle prepares iteration it + 1 (a no-op after the stop)
-                int t2 = _timer.begin_iter(_s, it + 1);
-                z = _mg->vcycle(_r.ptr, _st, rb);
-                _timer.end(t2, _s, TK_VCYCLE, it + 1);
-                cur ^= 1;
-            }
+        for (int c = 0; c < batch; ++c) {
+            const int it = it0 + launched + c + 1;
+            int t0 = _timer.begin_iter(_s, it);
+            launch_pcg_dir_spmv<double>(_g_unit, z, _p[cur].ptr, _p[cur ^ 1].ptr, _st, rb, _s);
+            _timer.end(t0, _s, TK_DIR_SPMV, it);
+            int t1 = _timer.begin_iter(_s, it);
+            launch_pcg_update<double>(_g, x, _r.ptr, _p[cur ^ 1].ptr, _st, rb, _s);
+            _timer.end(t1, _s, TK_UPDATE, it);
+            // the V-cycle prepares iteration it + 1 (a no-op after the stop)
+            int t2 = _timer.begin_iter(_s, it + 1);
+            z = _mg->vcycle(_r.ptr, _st, rb);
+            _timer.end(t2, _s, TK_VCYCLE, it + 1);
+            cur ^= 1;
         }
         launched += batch;
         readState(hs);

@@ -9,7 +9,6 @@
 #include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <array>
-#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -173,23 +172,6 @@ class VelocityCorrector
     double* scratch() const { return _scratch.ptr; }
 
   private:
-    // Iterations are queued as replays of captured HIP graphs, each
-    // kGraphIters iterations (an even count, so the direction buffers end
-    // where they started), unless per-kernel event timing is on.
-    static constexpr int kGraphIters = 8;
-    static constexpr int kGraphItersMg = 2;
-    struct CachedGraph
-    {
-        const void* key[6];
-        hipGraphExec_t exec;
-    };
-    std::vector<CachedGraph> _graphs;
-    // The graph of kGraphIters Jacobi-PCG iterations on these arrays
-    // (captured on first use), or of kGraphItersMg MG-PCG iterations.
-    template <class T>
-    hipGraphExec_t pcgGraph(T* x, T* r, DevArray<T> (&p)[2], int cur);
-    hipGraphExec_t mgGraph(double* x, const double* z, int cur);
-    hipGraphExec_t cachedGraph(const void* const (&key)[6], const std::function<void()>& body);
     cf_status solve64(const double* b, double* x, cf_solve_stats* st);
     cf_status solveRefined(const double* b, double* x, cf_solve_stats* st);
     cf_status solveRestart(const double* b, double* x, cf_solve_stats& loc);
