@@ -40,9 +40,10 @@ def kernels(obj):
             continue
         key, val = m.group(1), m.group(2)
         if key in ("group_segment_fixed_size", "private_segment_fixed_size", "vgpr_count",
-                   "name"):
+                   "vgpr_spill_count", "name"):
             cur[key] = val
-        if key == "symbol" and "name" in cur:
+        # a kernel's keys come in alphabetical order; .wavefront_size is its last
+        if key == "wavefront_size" and "name" in cur:
             out.append(cur)
             cur = {}
     return out
@@ -62,10 +63,11 @@ def main(argv):
     if "--table" in argv:
         path = argv[argv.index("--table") + 1]
         with open(path, "w") as f:
-            f.write("kernel\tlds_bytes\tscratch_bytes_per_lane\tvgprs\n")
+            f.write("kernel\tlds_bytes\tscratch_bytes_per_lane\tvgprs\tvgpr_spills\n")
             for k in ks:
                 f.write(f"{k['name']}\t{k['group_segment_fixed_size']}\t"
-                        f"{k.get('private_segment_fixed_size', '?')}\t{k.get('vgpr_count', '?')}\n")
+                        f"{k.get('private_segment_fixed_size', '?')}\t{k.get('vgpr_count', '?')}\t"
+                        f"{k.get('vgpr_spill_count', '?')}\n")
     if bad:
         return 1
     print(f"check_lds: {len(hits)} {name} kernels at {want} B of LDS")
