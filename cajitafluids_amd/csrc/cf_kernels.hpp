@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "cf_geometry.hpp"
 
 namespace cfmi
@@ -18,6 +20,7 @@ struct PcgState
     int mg; // z comes from the multigrid V-cycle: the update kernel leaves
             // beta and rho to k_mgpcg_rz
     int nonfinite; // ||b|| or ||r|| came out NaN or Inf: the solve stopped
+    double alpha_lag; // alpha of an X_SKIP update, applied by the next X_PAIR
 };
 
 // Grid-reduction scratch of the streaming kernels: per-block double-double
@@ -84,9 +87,33 @@ void launch_pcg_init32(const Geo& g, const float* r, float* d, PcgState* st,
 template <class T>
 void launch_pcg_dir_spmv(const Geo& g, const T* r, const T* p_old, T* p_new,
                          PcgState* st, RedBuf rb, hipStream_t s);
+// The update's treatment of x.  The 3-D FP64 tile kernels can leave x alone on
+// every second iteration and apply both steps on the next, in the order of
+// two single steps (same bits):
+//   X_EACH: x = fma(a_k, p_k, x)                                 5 words
+//   X_SKIP: x unchanged; the state keeps a_k as alpha_lag         3 words
+//   X_PAIR: x = fma(a_k, p_k, fma(alpha_lag, p_{k-1}, x))         6 words
+// p_{k-1} is the other direction buffer, which the direction kernel of
+// iteration k read and left intact.  A run that stops after an X_SKIP
+// update completes x with launch_pcg_xfix.
+enum XUpdate { X_EACH = 0, X_SKIP = 1, X_PAIR = 2 };
 template <class T>
 void launch_pcg_update(const Geo& g, T* x, T* r, const T* p, PcgState* st,
-                       RedBuf rb, hipStream_t s);
+                       RedBuf rb, hipStream_t s, int xmode = X_EACH,
+                       const T* p_lag = nullptr);
+// x = fma(alpha, p, x) over the owned cells with the state's alpha: the x
+// step of a run's last iteration when that was an X_SKIP update.
+template <class T>
+void launch_pcg_xfix(const Geo& g, T* x, const T* p, const PcgState* st, hipStream_t s);
+// Whether launch_pcg_update<T> takes X_SKIP / X_PAIR on this geometry: the
+// 3-D FP64 tile kernels do.  The 2-D kernels update x every iteration, and
+// so do the FP32 ones, whose X_PAIR variant would need 143 VGPRs (3 waves
+// per SIMD instead of 4).
+template <class T>
+inline bool pcg_pairs_x(const Geo& g)
+{
+    return g.dim == 3 && std::is_same<T, double>::value;
+}
 // Largest grid any reducing kernel of this geometry launches (sizes the
 // partials buffer).
 int reduction_blocks_max(const Geo& g);

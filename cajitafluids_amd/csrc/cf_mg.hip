@@ -693,6 +693,7 @@ __global__ __launch_bounds__(256) void k_mgpcg_init(Geo g, const double* __restr
         st->rho = 0.0;
         st->alpha = 0.0;
         st->beta = 0.0;
+        st->alpha_lag = 0.0;
         const bool zero = norm == 0.0;
         const bool bad = !isfinite(norm); // NaN or Inf in b: stop at once
         st->done = zero || bad ? 1 : (maxit <= 0 ? 1 : 0);

@@ -28,6 +28,7 @@
 // the host queues iterations in batches and polls once per batch.
 #include <hip/hip_runtime.h>
 
+#include "cf_solver.hpp"
 #include "cf_stream.hpp"
 #include "cf_tile3.hpp"
 
@@ -47,6 +48,7 @@ __device__ __forceinline__ void init_state(PcgState* st, double rz, double nn,
     st->rho = rz;
     st->alpha = 0.0;
     st->beta = 0.0;
+    st->alpha_lag = 0.0;
     const bool zero = norm == 0.0;
     // a NaN or Inf in the right-hand side stops the solve before it starts
     const bool bad = !isfinite(norm);
@@ -267,16 +269,21 @@ struct DirEpiT
     }
 };
 
-template <class T, int CPL>
+// The update's epilogue; XM is its x mode (XUpdate).
+template <class T, int CPL, int XM>
 struct UpdEpiT
 {
     const Geo& g;
     T* __restrict__ x;
     T* __restrict__ r;
     T alpha;
+    const T* __restrict__ p_lag; // X_PAIR: the previous direction
+    T alpha_lag;
     double hi[2] = {0.0, 0.0}, lo[2] = {0.0, 0.0};
-    // x and r of a plane, in two buffers that alternate with the march's steps
-    T xb[2][tile3::RY][CPL], rb[2][tile3::RY][CPL];
+    // x, r (and p_lag) of a plane, in two buffers that alternate with the
+    // march's steps
+    T xb[2][tile3::RY][CPL], rb[2][tile3::RY][CPL], lb[2][tile3::RY][CPL];
+    static constexpr bool kReadX = XM != X_SKIP;
     template <int B>
     __device__ void prefetch(const Tile3Ctx<CPL>& c, const Plane3Meta& m, int k)
     {
@@ -287,13 +294,19 @@ struct UpdEpiT
             const int64_t at = c.colb[q] + m.off;
             if constexpr (CPL == 1) {
                 // raw: cells() reads only valid rows of planes below k1
-                xb[B][q][0] = load_any<T>(x, ok, at, c.safe);
+                if constexpr (kReadX)
+                    xb[B][q][0] = load_any<T>(x, ok, at, c.safe);
+                if constexpr (XM == X_PAIR)
+                    lb[B][q][0] = load_any<T>(p_lag, ok, at, c.safe);
                 rb[B][q][0] = load_any<T>(r, ok, at, c.safe);
                 continue;
             }
             // an aligned pair on every path (raw, as above)
             const int64_t ap = ok ? at : c.safe;
-            vload<T, CPL>(x + ap, xb[B][q]);
+            if constexpr (kReadX)
+                vload<T, CPL>(x + ap, xb[B][q]);
+            if constexpr (XM == X_PAIR)
+                vload<T, CPL>(p_lag + ap, lb[B][q]);
             vload<T, CPL>(r + ap, rb[B][q]);
         }
     }
@@ -303,7 +316,10 @@ struct UpdEpiT
         for (int q = 0; q < tile3::RY; ++q)
 #pragma unroll
             for (int j = 0; j < CPL; ++j) {
-                xb[0][q][j] = xb[1][q][j];
+                if constexpr (kReadX)
+                    xb[0][q][j] = xb[1][q][j];
+                if constexpr (XM == X_PAIR)
+                    lb[0][q][j] = lb[1][q][j];
                 rb[0][q][j] = rb[1][q][j];
             }
     }
@@ -313,13 +329,21 @@ struct UpdEpiT
                           const T (&pc)[CPL], const T (&qv)[CPL])
     {
         const int64_t at = c.colb[q] + m.off;
-        T xo[CPL], ro[CPL];
+        T ro[CPL];
 #pragma unroll
-        for (int j = 0; j < CPL; ++j) {
-            xo[j] = fma_t<T>(alpha, pc[j], xb[B][q][j]);
+        for (int j = 0; j < CPL; ++j)
             ro[j] = fma_t<T>(-alpha, qv[j], rb[B][q][j]);
+        if constexpr (kReadX) {
+            T xo[CPL];
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) {
+                // X_PAIR: the skipped step first, then this one
+                const T xl = XM == X_PAIR ? fma_t<T>(alpha_lag, lb[B][q][j], xb[B][q][j])
+                                          : xb[B][q][j];
+                xo[j] = fma_t<T>(alpha, pc[j], xl);
+            }
+            vstore_valid<T, CPL>(x + at, xo, c.valid[q]);
         }
-        vstore_valid<T, CPL>(x + at, xo, c.valid[q]);
         vstore_valid<T, CPL>(r + at, ro, c.valid[q]);
 #pragma unroll
         for (int j = 0; j < CPL; ++j) {
@@ -350,24 +374,38 @@ __global__ __launch_bounds__(256) void k_pcg_dir_spmv3(
                    [&](double* sm) { st->alpha = st->rho / sm[0]; });
 }
 
-template <class T, int CPL, bool PX>
+template <class T, int CPL, bool PX, int XM>
 __global__ __launch_bounds__(256) void k_pcg_update3(
     Geo g, Tile3Grid tg, T* __restrict__ x, T* __restrict__ r,
-    const T* __restrict__ p, PcgState* st, RedBuf rb)
+    const T* __restrict__ p, const T* __restrict__ p_lag, PcgState* st, RedBuf rb)
 {
     using namespace tile3;
     if (st->done)
         return;
     __shared__ __align__(16) T pl[2][TY + 2][prow<CPL>()];
-    UpdEpiT<T, CPL> epi{g, x, r, (T)st->alpha};
+    UpdEpiT<T, CPL, XM> epi{g, x, r, (T)st->alpha, p_lag, (T)st->alpha_lag};
     // downwards: the direction kernel marches up, so each kernel starts on
     // the planes the other touched last (Infinity Cache hits; FP64 256^3:
     // update 127 -> 121 us, direction 101 -> 95 us)
     tile3_march<T, false, CPL, PX, true>(g, tg, p, nullptr, false, (T)0, pl, epi);
     dd_norm(epi.hi[0], epi.lo[0]);
     dd_norm(epi.hi[1], epi.lo[1]);
-    grid_reduce<2>(epi.hi, epi.lo, rb,
-                   [&](double* sm) { finish_update(st, sm[0], sm[1]); });
+    grid_reduce<2>(epi.hi, epi.lo, rb, [&](double* sm) {
+        if (XM == X_SKIP) // every block has read the state by now
+            st->alpha_lag = st->alpha;
+        finish_update(st, sm[0], sm[1]);
+    });
+}
+
+// x = fma(alpha, p, x): the x step an X_SKIP update left out, when it was
+// the last of a run (alpha is still that iteration's).
+template <class T>
+__global__ __launch_bounds__(256) void k_pcg_xfix(Geo g, T* __restrict__ x,
+                                                  const T* __restrict__ p,
+                                                  const PcgState* st)
+{
+    const T a = (T)st->alpha;
+    for_owned_cells(g, [&](int64_t c, int, int, int) { x[c] = fma_t<T>(a, p[c], x[c]); });
 }
 
 // ---- defect correction ---------------------------------------------------------
@@ -463,6 +501,7 @@ __global__ __launch_bounds__(256) void k_refine_init_carry(
         st->rho = s[2];
         st->alpha = 0.0;
         st->beta = 0.0;
+        st->alpha_lag = 0.0;
         st->mg = 0;
         // a non-finite ||b|| is reported by the host from rst->bnorm
         st->nonfinite = isfinite(bnorm) ? 0 : 1;
@@ -590,13 +629,36 @@ static void dir_spmv3_t(const Geo& g, const T* r, const T* p_old, T* p_new, PcgS
                        tile3_grid<cpl_of<T>()>(g), r, p_old, p_new, st, rb);
 }
 
-template <class T, bool PX>
-static void update3_t(const Geo& g, T* x, T* r, const T* p, PcgState* st, RedBuf rb,
-                      hipStream_t s)
+template <class T, bool PX, int XM>
+static void update3_t(const Geo& g, T* x, T* r, const T* p, const T* p_lag, PcgState* st,
+                      RedBuf rb, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_pcg_update3<T, cpl_of<T>(), PX>),
+    hipLaunchKernelGGL((k_pcg_update3<T, cpl_of<T>(), PX, XM>),
                        dim3(tile3_blocks<cpl_of<T>()>(g)), dim3(kBlock), 0, s, g,
-                       tile3_grid<cpl_of<T>()>(g), x, r, p, st, rb);
+                       tile3_grid<cpl_of<T>()>(g), x, r, p, p_lag, st, rb);
+}
+
+template <class T, bool PX>
+static void update3_x(const Geo& g, T* x, T* r, const T* p, const T* p_lag, PcgState* st,
+                      RedBuf rb, hipStream_t s, int xmode)
+{
+    if (xmode == X_EACH) {
+        update3_t<T, PX, X_EACH>(g, x, r, p, p_lag, st, rb, s);
+        return;
+    }
+    if constexpr (std::is_same<T, double>::value) {
+        if (xmode == X_SKIP) {
+            update3_t<T, PX, X_SKIP>(g, x, r, p, p_lag, st, rb, s);
+            return;
+        }
+        if (xmode == X_PAIR) {
+            if (!p_lag || p_lag == p)
+                throw CfError(CF_EINVAL, "X_PAIR update needs the previous direction buffer");
+            update3_t<T, PX, X_PAIR>(g, x, r, p, p_lag, st, rb, s);
+            return;
+        }
+    }
+    throw CfError(CF_EINVAL, "x mode not available for this PCG update");
 }
 
 template <class T>
@@ -617,18 +679,26 @@ void launch_pcg_dir_spmv(const Geo& g, const T* r, const T* p_old, T* p_new,
 
 template <class T>
 void launch_pcg_update(const Geo& g, T* x, T* r, const T* p, PcgState* st,
-                       RedBuf rb, hipStream_t s)
+                       RedBuf rb, hipStream_t s, int xmode, const T* p_lag)
 {
     if (g.dim == 2) {
+        if (xmode != X_EACH)
+            throw CfError(CF_EINVAL, "the 2-D PCG update takes x every iteration");
         CF_STREAM_LAUNCH((k_pcg_update<2, T>), g, x, r, p, st, rb);
         return;
     }
     if constexpr (cpl_of<T>() > 1)
         if (g.periodic[0]) {
-            update3_t<T, true>(g, x, r, p, st, rb, s);
+            update3_x<T, true>(g, x, r, p, p_lag, st, rb, s, xmode);
             return;
         }
-    update3_t<T, false>(g, x, r, p, st, rb, s);
+    update3_x<T, false>(g, x, r, p, p_lag, st, rb, s, xmode);
+}
+
+template <class T>
+void launch_pcg_xfix(const Geo& g, T* x, const T* p, const PcgState* st, hipStream_t s)
+{
+    CF_STREAM_LAUNCH(k_pcg_xfix<T>, g, x, p, st);
 }
 
 int reduction_blocks_max(const Geo& g)
@@ -644,9 +714,13 @@ template void launch_pcg_dir_spmv<float>(const Geo&, const float*, const float*,
                                          float*, PcgState*, RedBuf, hipStream_t);
 template void launch_pcg_update<double>(const Geo&, double*, double*,
                                         const double*, PcgState*, RedBuf,
-                                        hipStream_t);
+                                        hipStream_t, int, const double*);
 template void launch_pcg_update<float>(const Geo&, float*, float*, const float*,
-                                       PcgState*, RedBuf, hipStream_t);
+                                       PcgState*, RedBuf, hipStream_t, int, const float*);
+template void launch_pcg_xfix<double>(const Geo&, double*, const double*, const PcgState*,
+                                      hipStream_t);
+template void launch_pcg_xfix<float>(const Geo&, float*, const float*, const PcgState*,
+                                     hipStream_t);
 
 void launch_refine_init(const Geo& g, const double* b, double* x64,
                         float* r32, RefineState* rs, RedBuf rb, hipStream_t s)

@@ -274,6 +274,13 @@ void VelocityCorrector::applyPressure()
     launch_project(_g, _pm.fields(CF_CURRENT), _lhs.ptr, _s);
 }
 
+// The update's x mode for the n-th iteration (from 0) of a run: with pairs,
+// X_SKIP then X_PAIR, so that every run starts and ends on a whole x.
+static int xmode(bool pairs, int n)
+{
+    return !pairs ? X_EACH : (n & 1) ? X_PAIR : X_SKIP;
+}
+
 // Queue batches of iterations without waiting; poll the device flag once
 // per batch.  The first batch is sized from a prediction (the previous
 // solve's, or pass's, count) so that few no-op launches follow the stop.
@@ -289,6 +296,7 @@ int VelocityCorrector::runIterations(T* x, T* r, DevArray<T> (&p)[2], int& cur,
     if (hs.done)
         return hs.iter;
     const int it0 = hs.iter;
+    const bool pairs = pcg_pairs_x<T>(pcgGeo());
     int launched = 0;
     int batch = predict > 4 ? predict - 2 : 16;
     for (;;) {
@@ -298,7 +306,8 @@ int VelocityCorrector::runIterations(T* x, T* r, DevArray<T> (&p)[2], int& cur,
             launch_pcg_dir_spmv<T>(pcgGeo(), r, p[cur].ptr, p[cur ^ 1].ptr, _st, rb, _s);
             _timer.end(t0, _s, tk0 + 0, it);
             int t2 = _timer.begin_iter(_s, it);
-            launch_pcg_update<T>(pcgGeo(), x, r, p[cur ^ 1].ptr, _st, rb, _s);
+            launch_pcg_update<T>(pcgGeo(), x, r, p[cur ^ 1].ptr, _st, rb, _s,
+                                 xmode(pairs, launched + c), p[cur].ptr);
             _timer.end(t2, _s, tk0 + 1, it);
             cur ^= 1;
         }
@@ -311,7 +320,10 @@ int VelocityCorrector::runIterations(T* x, T* r, DevArray<T> (&p)[2], int& cur,
     }
     // The no-op launches after the stop still flipped `cur`: recover the
     // buffer that holds the last real direction.
-    cur ^= (launched - (hs.iter - it0)) & 1;
+    const int real = hs.iter - it0;
+    cur ^= (launched - real) & 1;
+    if (pairs && (real & 1)) // the last real update was an X_SKIP
+        launch_pcg_xfix<T>(pcgGeo(), x, p[cur].ptr, _st, _s);
     return hs.iter;
 }
 
@@ -523,6 +535,7 @@ cf_status VelocityCorrector::solveMg(const double* b, double* x, cf_solve_stats*
     int cur = 0, launched = 0;
     int batch = _predict_mg > 3 ? _predict_mg - 2 : 8;
     const int it0 = hs.iter;
+    const bool pairs = pcg_pairs_x<double>(_g);
     while (!hs.done) {
         for (int c = 0; c < batch; ++c) {
             const int it = it0 + launched + c + 1;
@@ -530,7 +543,8 @@ cf_status VelocityCorrector::solveMg(const double* b, double* x, cf_solve_stats*
             launch_pcg_dir_spmv<double>(_g_unit, z, _p[cur].ptr, _p[cur ^ 1].ptr, _st, rb, _s);
             _timer.end(t0, _s, TK_DIR_SPMV, it);
             int t1 = _timer.begin_iter(_s, it);
-            launch_pcg_update<double>(_g, x, _r.ptr, _p[cur ^ 1].ptr, _st, rb, _s);
+            launch_pcg_update<double>(_g, x, _r.ptr, _p[cur ^ 1].ptr, _st, rb, _s,
+                                      xmode(pairs, launched + c), _p[cur].ptr);
             _timer.end(t1, _s, TK_UPDATE, it);
             // the V-cycle prepares iteration it + 1 (a no-op after the stop)
             int t2 = _timer.begin_iter(_s, it + 1);
@@ -542,6 +556,11 @@ cf_status VelocityCorrector::solveMg(const double* b, double* x, cf_solve_stats*
         readState(hs);
         _timer.resolve(hs.done ? hs.iter + 1 : INT_MAX);
         batch = 2;
+    }
+    const int real = hs.iter - it0;
+    if (pairs && (real & 1)) { // the last real update was an X_SKIP
+        cur ^= (launched - real) & 1;
+        launch_pcg_xfix<double>(_g, x, _p[cur].ptr, _st, _s);
     }
     // On a singular operator the pressure is fixed up to a constant, which
     // MG-PCG leaves at a level of the V-cycle's (only the as-written FaceJ

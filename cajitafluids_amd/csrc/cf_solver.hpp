@@ -86,13 +86,16 @@ class KernelTimer
   public:
     ~KernelTimer();
     bool enabled = false;
-    int sample = 1; // PCG kernels: time one iteration in `sample`
+    int sample = 1; // PCG kernels: time two consecutive iterations in `sample`
     // Record a start event; returns a slot for end().
     int begin(hipStream_t s);
-    // begin() for a kernel of PCG iteration `it` (sampled).
+    // begin() for a kernel of PCG iteration `it` (1-based; sampled).  The
+    // samples come in pairs of iterations so that the update's X_SKIP and
+    // X_PAIR variants, which alternate, are timed alike.
     int begin_iter(hipStream_t s, int it)
     {
-        return (sample <= 1 || it % sample == 0) ? begin(s) : -1;
+        const int half = sample / 2 > 1 ? sample / 2 : 1;
+        return (sample <= 1 || ((it - 1) / 2) % half == 0) ? begin(s) : -1;
     }
     void end(int slot, hipStream_t s, int kernel, int iter);
     // Resolve pending pairs (stream must be synchronised); pairs whose tag is
