@@ -20,17 +20,15 @@ sys.path.insert(0, REPO)
 METRIC = "cell-steps/s at 256^3 + PCG HBM GB/s; FP32-refined vs FP64 at 1e-8 resid"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic vector words per owned cell per launch of each PCG kernel:
-# dir_spmv reads r and the old direction and writes the new one (3).  The
-# FP64 update alternates two variants: X_SKIP reads p, r and writes r (3);
-# X_PAIR reads x, p, the previous p, r and writes x, r (6) -- 4.5 words per
-# launch on average, 7.5 per iteration.  The FP32 update reads x, p, r and
-# writes x, r every iteration (5).  The reference-shaped split (SpMV /
-# update / direction) moves 11 words per iteration.  "pcg.gbs_11_words"
-# counts 11 words per iteration (the contract's convention), the roofline
-# object counts each kernel's own words.
-WORDS64 = {"dir_spmv": 3, "update": 4.5}
-WORDS32 = {"dir_spmv": 3, "update": 5}
+# Algorithmic vector words per owned cell per launch of each 3-D PCG kernel
+# (8 bytes in FP64, 4 in FP32): dir_spmv reads r and the old direction and
+# writes the new one (3).  The update alternates two variants: X_SKIP reads
+# p, r and writes r (3); X_PAIR reads x, p, the previous p, r and writes x,
+# r (6) -- 4.5 words per launch on average, 7.5 per iteration.  The
+# reference-shaped split (SpMV / update / direction) moves 11 words per
+# iteration.  "pcg.gbs_11_words" counts 11 words per iteration (the
+# contract's convention), the roofline object counts each kernel's own words.
+WORDS = {"dir_spmv": 3, "update": 4.5}
 # rows of the kernel-trace / counter tables for each timed FP64 kernel
 KERNEL64 = {"dir_spmv": "k_pcg_dir_spmv3<double, 1>",
             "update": "k_pcg_update3<double, 1> (X_SKIP and X_PAIR, mean)"}
@@ -270,7 +268,7 @@ def main():
         ms, cnt = kt[k]
         if cnt:
             per[nm] = dict(avg_us=1e3 * ms / cnt, launches=cnt, total_ms=ms,
-                           bytes=WORDS64[nm] * 8 * ncell)
+                           bytes=WORDS[nm] * 8 * ncell)
     # Kernel time per iteration from the sampled launches, times the
     # iterations of the timed region.
     t_pcg = sum(p["avg_us"] for p in per.values()) * 1e-6 * sum(it64)
@@ -331,7 +329,7 @@ def main():
         for k, nm in ((_lib.TK_DIR_SPMV32, "dir_spmv"), (_lib.TK_UPDATE32, "update")):
             ms, cnt = kt32[k]
             if cnt:
-                gbs = WORDS32[nm] * 4 * ncell / (ms / cnt * 1e-3) / 1e9
+                gbs = WORDS[nm] * 4 * ncell / (ms / cnt * 1e-3) / 1e9
                 k32[nm] = {"avg_us": round(1e3 * ms / cnt, 3), "launches": cnt,
                            "gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
         ms, cnt = kt32[_lib.TK_RESIDUAL]

@@ -87,7 +87,7 @@ void launch_pcg_init32(const Geo& g, const float* r, float* d, PcgState* st,
 template <class T>
 void launch_pcg_dir_spmv(const Geo& g, const T* r, const T* p_old, T* p_new,
                          PcgState* st, RedBuf rb, hipStream_t s);
-// The update's treatment of x.  The 3-D FP64 tile kernels can leave x alone on
+// The update's treatment of x.  The 3-D tile kernels can leave x alone on
 // every second iteration and apply both steps on the next, in the order of
 // two single steps (same bits):
 //   X_EACH: x = fma(a_k, p_k, x)                                 5 words
@@ -106,13 +106,11 @@ void launch_pcg_update(const Geo& g, T* x, T* r, const T* p, PcgState* st,
 template <class T>
 void launch_pcg_xfix(const Geo& g, T* x, const T* p, const PcgState* st, hipStream_t s);
 // Whether launch_pcg_update<T> takes X_SKIP / X_PAIR on this geometry: the
-// 3-D FP64 tile kernels do.  The 2-D kernels update x every iteration, and
-// so do the FP32 ones, whose X_PAIR variant would need 143 VGPRs (3 waves
-// per SIMD instead of 4).
+// 3-D tile kernels do; the 2-D kernels update x every iteration.
 template <class T>
 inline bool pcg_pairs_x(const Geo& g)
 {
-    return g.dim == 3 && std::is_same<T, double>::value;
+    return g.dim == 3;
 }
 // Largest grid any reducing kernel of this geometry launches (sizes the
 // partials buffer).

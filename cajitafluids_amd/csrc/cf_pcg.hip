@@ -23,6 +23,9 @@
 // 8 vector words per cell per iteration instead of the 11 of the
 // SpMV / update / direction split.  p is double-buffered because dir_spmv
 // reads the old direction around each cell while writing the new one.  The
+// 3-D FP64 update takes x every second iteration (XUpdate: the skipped step
+// and the current one are applied together, in order, from the two
+// direction buffers), 7.5 words per iteration.  The
 // last block of each kernel to finish computes the scalars on the device;
 // every kernel returns at once when the device-side `done` flag is set, so
 // the host queues iterations in batches and polls once per batch.
@@ -255,7 +258,7 @@ struct DirEpiT
     __device__ void prefetch(const Tile3Ctx<CPL>&, const Plane3Meta&, int)
     {
     }
-    __device__ void shift() {}
+    __device__ void shift(const Tile3Ctx<CPL>&) {}
     __device__ void rotate() {}
     template <int B>
     __device__ void cells(const Tile3Ctx<CPL>& c, const Plane3Meta& m, int q,
@@ -270,6 +273,14 @@ struct DirEpiT
 };
 
 // The update's epilogue; XM is its x mode (XUpdate).
+//
+// With one cell per lane, x, r (and the previous direction) of the next
+// plane load into the second of two buffers while the march works on the
+// current one.  With two cells per lane the march keeps one buffer pair and
+// copies after each plane (shift); there the X_PAIR variant loads x, r and
+// the previous direction of the next plane in shift() into single "late"
+// buffers instead, which keeps it at 4 waves per SIMD (double-buffered it
+// would take 143 VGPRs, 3 waves).
 template <class T, int CPL, int XM>
 struct UpdEpiT
 {
@@ -280,13 +291,39 @@ struct UpdEpiT
     const T* __restrict__ p_lag; // X_PAIR: the previous direction
     T alpha_lag;
     double hi[2] = {0.0, 0.0}, lo[2] = {0.0, 0.0};
+    static constexpr bool kReadX = XM != X_SKIP;
+    static constexpr bool kLate = XM == X_PAIR && CPL > 1;
     // x, r (and p_lag) of a plane, in two buffers that alternate with the
     // march's steps
     T xb[2][tile3::RY][CPL], rb[2][tile3::RY][CPL], lb[2][tile3::RY][CPL];
-    static constexpr bool kReadX = XM != X_SKIP;
+    // kLate: x, r and p_lag of the current plane, and where the next one is
+    T xs[tile3::RY][CPL], rs[tile3::RY][CPL], ls[tile3::RY][CPL];
+    int next_k = 0;
+    int64_t next_off = 0;
+
+    __device__ void load_late(const Tile3Ctx<CPL>& c, int64_t off, int k)
+    {
+#pragma unroll
+        for (int q = 0; q < tile3::RY; ++q) {
+            const bool ok = k >= c.k0 && k < c.k1 && c.valid[q][0];
+            const int64_t ap = ok ? c.colb[q] + off : c.safe;
+            vload<T, CPL>(x + ap, xs[q]);
+            vload<T, CPL>(p_lag + ap, ls[q]);
+            vload<T, CPL>(r + ap, rs[q]);
+        }
+    }
     template <int B>
     __device__ void prefetch(const Tile3Ctx<CPL>& c, const Plane3Meta& m, int k)
     {
+        if constexpr (kLate) {
+            if (B == 0) { // the march's first plane: load now
+                load_late(c, m.off, k);
+            } else { // the next plane: shift() loads it
+                next_k = k;
+                next_off = m.off;
+            }
+            return;
+        }
 #pragma unroll
         for (int q = 0; q < tile3::RY; ++q) {
             // a group with a valid first cell lies inside its padded row
@@ -305,23 +342,23 @@ struct UpdEpiT
             const int64_t ap = ok ? at : c.safe;
             if constexpr (kReadX)
                 vload<T, CPL>(x + ap, xb[B][q]);
-            if constexpr (XM == X_PAIR)
-                vload<T, CPL>(p_lag + ap, lb[B][q]);
             vload<T, CPL>(r + ap, rb[B][q]);
         }
     }
-    __device__ void shift()
+    __device__ void shift(const Tile3Ctx<CPL>& c)
     {
 #pragma unroll
         for (int q = 0; q < tile3::RY; ++q)
 #pragma unroll
             for (int j = 0; j < CPL; ++j) {
+                if constexpr (kLate)
+                    continue;
                 if constexpr (kReadX)
                     xb[0][q][j] = xb[1][q][j];
-                if constexpr (XM == X_PAIR)
-                    lb[0][q][j] = lb[1][q][j];
                 rb[0][q][j] = rb[1][q][j];
             }
+        if constexpr (kLate)
+            load_late(c, next_off, next_k);
     }
     __device__ void rotate() {}
     template <int B>
@@ -332,14 +369,15 @@ struct UpdEpiT
         T ro[CPL];
 #pragma unroll
         for (int j = 0; j < CPL; ++j)
-            ro[j] = fma_t<T>(-alpha, qv[j], rb[B][q][j]);
+            ro[j] = fma_t<T>(-alpha, qv[j], kLate ? rs[q][j] : rb[B][q][j]);
         if constexpr (kReadX) {
             T xo[CPL];
 #pragma unroll
             for (int j = 0; j < CPL; ++j) {
+                const T xv = kLate ? xs[q][j] : xb[B][q][j];
+                const T pv = kLate ? ls[q][j] : lb[B][q][j];
                 // X_PAIR: the skipped step first, then this one
-                const T xl = XM == X_PAIR ? fma_t<T>(alpha_lag, lb[B][q][j], xb[B][q][j])
-                                          : xb[B][q][j];
+                const T xl = XM == X_PAIR ? fma_t<T>(alpha_lag, pv, xv) : xv;
                 xo[j] = fma_t<T>(alpha, pc[j], xl);
             }
             vstore_valid<T, CPL>(x + at, xo, c.valid[q]);
@@ -646,7 +684,7 @@ static void update3_x(const Geo& g, T* x, T* r, const T* p, const T* p_lag, PcgS
         update3_t<T, PX, X_EACH>(g, x, r, p, p_lag, st, rb, s);
         return;
     }
-    if constexpr (std::is_same<T, double>::value) {
+    {
         if (xmode == X_SKIP) {
             update3_t<T, PX, X_SKIP>(g, x, r, p, p_lag, st, rb, s);
             return;

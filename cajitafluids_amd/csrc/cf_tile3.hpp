@@ -592,7 +592,8 @@ __device__ __forceinline__ T tile3_apply(const Geo& g, const Tile3Ctx<CPL>& c,
 // Epilogue interface: prefetch<B>(c, m, k) loads plane k's data into buffer
 // B (k outside [k0, k1): values never used), take_raw<B>(P) takes the marched operand
 // instead, cells<B>() uses buffer B, shift() copies buffer 1 into 0 (CPL = 2
-// only), rotate() runs after every step's barrier.
+// only; it may issue loads for the next plane), rotate() runs after every
+// step's barrier.
 //
 // REV marches the chunk downwards (k1-1 to k0).  A kernel that follows one
 // marching the other way then starts on the planes its predecessor touched
@@ -715,7 +716,7 @@ __device__ __forceinline__ void tile3_march(
             P0 = P1;
             mc = mn;
             mn = mnn;
-            epi.shift();
+            epi.shift(c);
             epi.rotate();
         }
     }
