@@ -258,7 +258,7 @@ struct DirEpiT
     __device__ void prefetch(const Tile3Ctx<CPL>&, const Plane3Meta&, int)
     {
     }
-    __device__ void shift(const Tile3Ctx<CPL>&) {}
+    __device__ void shift() {}
     __device__ void rotate() {}
     template <int B>
     __device__ void cells(const Tile3Ctx<CPL>& c, const Plane3Meta& m, int q,
@@ -274,13 +274,12 @@ struct DirEpiT
 
 // The update's epilogue; XM is its x mode (XUpdate).
 //
-// With one cell per lane, x, r (and the previous direction) of the next
-// plane load into the second of two buffers while the march works on the
-// current one.  With two cells per lane the march keeps one buffer pair and
-// copies after each plane (shift); there the X_PAIR variant loads x, r and
-// the previous direction of the next plane in shift() into single "late"
-// buffers instead, which keeps it at 4 waves per SIMD (double-buffered it
-// would take 143 VGPRs, 3 waves).
+// X_EACH and X_SKIP: x and r of the next plane load into the second of two
+// buffers (prefetch) while the march works on the current one.  X_PAIR
+// reads one more vector; it loads x, r and the previous direction of the
+// next plane after each step's barrier instead (kLate), into one buffer,
+// which keeps it at 4 waves per SIMD in FP32 (double-buffered it would take
+// 143 VGPRs, 3 waves).
 template <class T, int CPL, int XM>
 struct UpdEpiT
 {
@@ -292,21 +291,29 @@ struct UpdEpiT
     T alpha_lag;
     double hi[2] = {0.0, 0.0}, lo[2] = {0.0, 0.0};
     static constexpr bool kReadX = XM != X_SKIP;
-    static constexpr bool kLate = XM == X_PAIR && CPL > 1;
-    // x, r (and p_lag) of a plane, in two buffers that alternate with the
-    // march's steps
-    T xb[2][tile3::RY][CPL], rb[2][tile3::RY][CPL], lb[2][tile3::RY][CPL];
+    static constexpr bool kLate = XM == X_PAIR;
+    // x and r of a plane, in two buffers that alternate with the march's
+    // steps (X_EACH, X_SKIP)
+    T xb[2][tile3::RY][CPL], rb[2][tile3::RY][CPL];
     // kLate: x, r and p_lag of the current plane, and where the next one is
     T xs[tile3::RY][CPL], rs[tile3::RY][CPL], ls[tile3::RY][CPL];
     int next_k = 0;
     int64_t next_off = 0;
 
-    __device__ void load_late(const Tile3Ctx<CPL>& c, int64_t off, int k)
+    __device__ void late(const Tile3Ctx<CPL>& c)
     {
 #pragma unroll
         for (int q = 0; q < tile3::RY; ++q) {
-            const bool ok = k >= c.k0 && k < c.k1 && c.valid[q][0];
-            const int64_t ap = ok ? c.colb[q] + off : c.safe;
+            // a group with a valid first cell lies inside its padded row
+            const bool ok = next_k >= c.k0 && next_k < c.k1 && c.valid[q][0];
+            const int64_t at = c.colb[q] + next_off;
+            if constexpr (CPL == 1) {
+                xs[q][0] = load_any<T>(x, ok, at, c.safe);
+                ls[q][0] = load_any<T>(p_lag, ok, at, c.safe);
+                rs[q][0] = load_any<T>(r, ok, at, c.safe);
+                continue;
+            }
+            const int64_t ap = ok ? at : c.safe;
             vload<T, CPL>(x + ap, xs[q]);
             vload<T, CPL>(p_lag + ap, ls[q]);
             vload<T, CPL>(r + ap, rs[q]);
@@ -315,13 +322,9 @@ struct UpdEpiT
     template <int B>
     __device__ void prefetch(const Tile3Ctx<CPL>& c, const Plane3Meta& m, int k)
     {
-        if constexpr (kLate) {
-            if (B == 0) { // the march's first plane: load now
-                load_late(c, m.off, k);
-            } else { // the next plane: shift() loads it
-                next_k = k;
-                next_off = m.off;
-            }
+        if constexpr (kLate) { // late() loads it
+            next_k = k;
+            next_off = m.off;
             return;
         }
 #pragma unroll
@@ -333,8 +336,6 @@ struct UpdEpiT
                 // raw: cells() reads only valid rows of planes below k1
                 if constexpr (kReadX)
                     xb[B][q][0] = load_any<T>(x, ok, at, c.safe);
-                if constexpr (XM == X_PAIR)
-                    lb[B][q][0] = load_any<T>(p_lag, ok, at, c.safe);
                 rb[B][q][0] = load_any<T>(r, ok, at, c.safe);
                 continue;
             }
@@ -345,20 +346,18 @@ struct UpdEpiT
             vload<T, CPL>(r + ap, rb[B][q]);
         }
     }
-    __device__ void shift(const Tile3Ctx<CPL>& c)
+    __device__ void shift()
     {
+        if constexpr (!kLate) {
 #pragma unroll
-        for (int q = 0; q < tile3::RY; ++q)
+            for (int q = 0; q < tile3::RY; ++q)
 #pragma unroll
-            for (int j = 0; j < CPL; ++j) {
-                if constexpr (kLate)
-                    continue;
-                if constexpr (kReadX)
-                    xb[0][q][j] = xb[1][q][j];
-                rb[0][q][j] = rb[1][q][j];
-            }
-        if constexpr (kLate)
-            load_late(c, next_off, next_k);
+                for (int j = 0; j < CPL; ++j) {
+                    if constexpr (kReadX)
+                        xb[0][q][j] = xb[1][q][j];
+                    rb[0][q][j] = rb[1][q][j];
+                }
+        }
     }
     __device__ void rotate() {}
     template <int B>
@@ -374,10 +373,8 @@ struct UpdEpiT
             T xo[CPL];
 #pragma unroll
             for (int j = 0; j < CPL; ++j) {
-                const T xv = kLate ? xs[q][j] : xb[B][q][j];
-                const T pv = kLate ? ls[q][j] : lb[B][q][j];
                 // X_PAIR: the skipped step first, then this one
-                const T xl = XM == X_PAIR ? fma_t<T>(alpha_lag, pv, xv) : xv;
+                const T xl = kLate ? fma_t<T>(alpha_lag, ls[q][j], xs[q][j]) : xb[B][q][j];
                 xo[j] = fma_t<T>(alpha, pc[j], xl);
             }
             vstore_valid<T, CPL>(x + at, xo, c.valid[q]);

@@ -99,6 +99,19 @@ struct epi_takes_raw<E, std::void_t<decltype(E::kRaw)>> : std::bool_constant<E::
 {
 };
 
+// An epilogue with kLate = true loads its data of a plane itself, in
+// late(c): the march calls it after the prologue and after every step's
+// barrier, for the plane last passed to prefetch (which then only records
+// the plane).  One buffer then holds that data instead of two.
+template <class E, class = void>
+struct epi_loads_late : std::false_type
+{
+};
+template <class E>
+struct epi_loads_late<E, std::void_t<decltype(E::kLate)>> : std::bool_constant<E::kLate>
+{
+};
+
 template <int CPL>
 inline Tile3Grid tile3_grid(const Geo& g)
 {
@@ -592,8 +605,8 @@ __device__ __forceinline__ T tile3_apply(const Geo& g, const Tile3Ctx<CPL>& c,
 // Epilogue interface: prefetch<B>(c, m, k) loads plane k's data into buffer
 // B (k outside [k0, k1): values never used), take_raw<B>(P) takes the marched operand
 // instead, cells<B>() uses buffer B, shift() copies buffer 1 into 0 (CPL = 2
-// only; it may issue loads for the next plane), rotate() runs after every
-// step's barrier.
+// only), rotate() runs after every step's barrier; late(c) as described at
+// epi_loads_late.
 //
 // REV marches the chunk downwards (k1-1 to k0).  A kernel that follows one
 // marching the other way then starts on the planes its predecessor touched
@@ -640,6 +653,8 @@ __device__ __forceinline__ void tile3_march(
     if constexpr (epi_takes_raw<E>::value)
         epi.template take_raw<0>(P1);
     epi.template prefetch<0>(c, mc, kbeg);
+    if constexpr (epi_loads_late<E>::value)
+        epi.late(c);
     epi.rotate();
     __syncthreads();
     auto step = [&](int k, auto parity) {
@@ -674,6 +689,8 @@ __device__ __forceinline__ void tile3_march(
             }
         }
         __syncthreads();
+        if constexpr (epi_loads_late<E>::value)
+            epi.late(c);
         mc = mn;
         mn = mnn;
         epi.rotate();
@@ -712,11 +729,13 @@ __device__ __forceinline__ void tile3_march(
                 }
             }
             __syncthreads();
+            if constexpr (epi_loads_late<E>::value)
+                epi.late(c);
             cur ^= 1;
             P0 = P1;
             mc = mn;
             mn = mnn;
-            epi.shift(c);
+            epi.shift();
             epi.rotate();
         }
     }
