@@ -4,13 +4,14 @@ Makefile after the library links).
 Reads the gfx950 code object inside an object file built here (the
 .hip_fatbin section, unbundled with clang-offload-bundler) and the kernel
 descriptors' metadata (llvm-readelf --notes), and fails unless every kernel
-whose name contains NAME has .group_segment_fixed_size == BYTES.  The 3-D
+whose (mangled) name matches the regular expression NAME has
+.group_segment_fixed_size == BYTES.  The 3-D
 advection adjoint declares exactly its LDS windows; any more (the compiler
 placing private arrays in LDS beside them) has given wrong, run-to-run
 varying gradients.
 
 usage: python3 check_lds.py OBJECT NAME BYTES [--table OUT]
-  --table OUT: also write every kernel's LDS, scratch and VGPR counts.
+  --table OUT: also write every kernel's LDS, scratch, VGPR and spill counts.
 """
 import os
 import re
@@ -52,7 +53,7 @@ def kernels(obj):
 def main(argv):
     obj, name, want = argv[1], argv[2], int(argv[3])
     ks = kernels(obj)
-    hits = [k for k in ks if name in k["name"]]
+    hits = [k for k in ks if re.search(name, k["name"])]
     if not hits:
         print(f"check_lds: no kernel named *{name}* in {obj}", file=sys.stderr)
         return 1
