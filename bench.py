@@ -243,7 +243,10 @@ def main():
         copy_probe()
     n = args.n
     if args.mg_only:
-        print(json.dumps({"multigrid": mg_leg(n, args, None, None, None)}))
+        m64 = mg_leg(n, args, None, None, None)
+        m32 = mg_leg(n, args, None, None, None, f32=True)
+        m32["speedup_vs_mg_fp64"] = round(m32["value"] / m64["value"], 3)
+        print(json.dumps({"multigrid": m64, "multigrid_fp32": m32}))
         return
     ncell = n ** 3
     if args.no_fp64:
@@ -350,6 +353,9 @@ def main():
 
     if not args.no_mg:
         out["multigrid"] = mg_leg(n, args, value, q64, p64)
+        out["multigrid_fp32"] = mg_leg(n, args, value, q64, p64, f32=True)
+        out["multigrid_fp32"]["speedup_vs_mg_fp64"] = round(
+            out["multigrid_fp32"]["value"] / out["multigrid"]["value"], 3)
 
     if args.adjoint:
         out["adjoint"] = adjoint_legs()
@@ -462,11 +468,12 @@ def dye_tank_leg(with_cpu):
     return out
 
 
-def mg_leg(n, args, value64, q64, p64):
-    """The same workload with the V-cycle preconditioner (FP64, same
-    1e-8 relative residual)."""
+def mg_leg(n, args, value64, q64, p64, f32=False):
+    """The same workload with the V-cycle preconditioner (same 1e-8
+    relative residual): FP64, or (f32) with the V-cycle's level 0 on FP32
+    vectors inside the FP64 CG (precision CF_FP32_REFINED)."""
     from cajitafluids_amd import _lib, configs as K
-    cfg = K.vortex_tank_3d(n)
+    cfg = K.vortex_tank_3d(n, precision=K.FP32_REFINED if f32 else K.FP64)
     cfg["preconditioner"] = K.PRECOND_MG
     for key, val in (("mg_nu", args.mg_nu), ("mg_alpha", args.mg_alpha),
                      ("mg_omega", args.mg_omega), ("mg_coarse_sweeps", args.mg_coarse)):
@@ -489,6 +496,7 @@ def mg_leg(n, args, value64, q64, p64):
             kern[nm] = {"avg_us": round(1e3 * ms / cnt, 3), "launches": cnt}
     return {
         "value": v, "unit": "cell-steps/s",
+        "vcycle_level0": "fp32" if f32 else "fp64",
         "speedup_vs_jacobi": round(v / value64, 3) if value64 else None,
         "ms_per_step": 1e3 * t / args.steps,
         "iterations_per_step": round(sum(its) / args.steps, 1),

@@ -58,8 +58,12 @@ cf_status build_geometry(const cf_config& c, Geo& g, const char** why)
         return CF_EINVAL;
     }
     if (c.preconditioner == CF_PRECOND_MG) {
-        if (c.precision != CF_FP64) {
-            *why = "the multigrid preconditioner runs in FP64 only";
+        // FP32_REFINED: the V-cycle's level 0 in FP32, on 3-D grids with
+        // more than 2 cells along every axis (level 0 marched by the tiles)
+        if (c.precision == CF_FP32_REFINED &&
+            !(c.dim == 3 && c.num_cell[0] > 2 && c.num_cell[1] > 2 && c.num_cell[2] > 2)) {
+            *why = "the FP32 multigrid V-cycle needs a 3-D grid with more than 2 cells "
+                   "along every axis";
             return CF_ENOTIMPL;
         }
         if (c.mg_nu < 1 || c.mg_coarse_sweeps < 1 || !(c.mg_omega > 0) ||

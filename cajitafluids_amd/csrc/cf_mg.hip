@@ -13,7 +13,10 @@
 // restriction, and the last level-0 sweep also reduces r.z for the PCG.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "cf_mg.hpp"
+#include "cf_solver.hpp"
 #include "cf_stream.hpp"
 #include "cf_tile3.hpp"
 
@@ -40,9 +43,10 @@ __device__ __forceinline__ int wrap_nbr(int periodic, int n, int i, int dir)
     return m;
 }
 
-template <int DIM>
-__global__ __launch_bounds__(256) void k_mg0_first(Geo g, MgTab wd, const double* __restrict__ b,
-                                                   double* __restrict__ x, const PcgState* st)
+// x = w b (w rounded to T on an FP32 level)
+template <int DIM, class T = double>
+__global__ __launch_bounds__(256) void k_mg0_first(Geo g, MgTab wd, const T* __restrict__ b,
+                                                   T* __restrict__ x, const PcgState* st)
 {
     if (st->done)
         return;
@@ -50,8 +54,20 @@ __global__ __launch_bounds__(256) void k_mg0_first(Geo g, MgTab wd, const double
         const int ub = wall_count(g, 1, j) + wall_count(g, 2, k);
         for (int i = threadIdx.x; i < g.n[0]; i += kBlock) {
             const int64_t c = base + i;
-            x[c] = tab_sel(wd, ub + wall_count(g, 0, i)) * b[c];
+            x[c] = (T)tab_sel(wd, ub + wall_count(g, 0, i)) * b[c];
         }
+    });
+}
+
+// b32 = (float) r over the owned cells: the FP32 level 0's right-hand side
+__global__ __launch_bounds__(256) void k_mg_cast32(Geo g, const double* __restrict__ r,
+                                                   float* __restrict__ b32, const PcgState* st)
+{
+    if (st->done)
+        return;
+    for_owned_rows(g, [&](int64_t base, int, int) {
+        for (int i = threadIdx.x; i < g.n[0]; i += kBlock)
+            b32[base + i] = (float)r[base + i];
     });
 }
 
@@ -141,22 +157,29 @@ __global__ __launch_bounds__(256) void k_mg0_smooth(Geo g, MgTab wd, const doubl
 
 // Sweep epilogue: xo = fma(w, b - y, x) at each own cell, w = omega / diag
 // from gw's Jacobi table; RZ also accumulates r.z = b . xo.  RAW: b is the
-// marched operand (the first sweep), taken from the tile's loads.
-template <bool RZ, bool RAW>
+// marched operand (the first sweep), taken from the tile's loads.  T is the
+// level's vector type.  On an FP32 level the RZ sweep (level 0's last) reads
+// the FP64 right-hand side r and writes z in FP64 (TB = double): the sweep
+// computes in FP32 from (float) r, and r.z uses r itself.
+template <class T, bool RZ>
+using MgRhsT = std::conditional_t<std::is_same<T, float>::value && RZ, double, T>;
+
+template <class T, bool RZ, bool RAW>
 struct MgSweepEpi
 {
+    using TB = MgRhsT<T, RZ>;
     static constexpr bool kRaw = RAW;
     const Geo& gw;
-    const double* __restrict__ b;
-    double* __restrict__ xo;
+    const TB* __restrict__ b;
+    TB* __restrict__ xo;
     double hi[1] = {0.0}, lo[1] = {0.0};
-    double bb[2][tile3::RY]; // b of a plane, alternating with the march's steps
+    TB bb[2][tile3::RY]; // b of a plane, alternating with the march's steps
     template <int B, class PL>
     __device__ void take_raw(const PL& P)
     {
 #pragma unroll
         for (int q = 0; q < tile3::RY; ++q)
-            bb[B][q] = P.a[q][0];
+            bb[B][q] = (TB)P.a[q][0];
     }
     template <int B>
     __device__ void prefetch(const Tile3Ctx<1>& c, const Plane3Meta& m, int k)
@@ -165,41 +188,44 @@ struct MgSweepEpi
 #pragma unroll
             for (int q = 0; q < tile3::RY; ++q) {
                 const bool ok = k < c.k1 && c.valid[q][0];
-                bb[B][q] = load_any<double>(b, ok, c.colb[q] + m.off, c.safe); // raw: read on valid rows only
+                bb[B][q] = load_any<TB>(b, ok, c.colb[q] + m.off, c.safe); // raw: read on valid rows only
             }
         }
     }
     __device__ void rotate() {}
     template <int B>
-    __device__ void cells(const Tile3Ctx<1>& c, const Plane3Meta& m, int q, const double (&pc)[1],
-                          const double (&y)[1])
+    __device__ void cells(const Tile3Ctx<1>& c, const Plane3Meta& m, int q, const T (&pc)[1],
+                          const T (&y)[1])
     {
-        const double w = dinv_pick<double>(gw, c.wy[q] + m.wz, c.wx[0]);
-        const double out = fma(w, bb[B][q] - y[0], pc[0]);
-        xo[c.colb[q] + m.off] = out;
+        const T w = dinv_pick<T>(gw, c.wy[q] + m.wz, c.wx[0]);
+        const T out = fma_t<T>(w, (T)bb[B][q] - y[0], pc[0]);
+        xo[c.colb[q] + m.off] = (TB)out;
         if constexpr (RZ)
-            dd_acc_prod<false>(hi[0], lo[0], bb[B][q], out);
+            dd_acc_prod<false>(hi[0], lo[0], (double)bb[B][q], (double)out);
     }
 };
 
 // MODE: 1 = first sweep from zero (values w b, read from b through gw's
 // Jacobi table), 0 = plain, 2 = after the coarse correction.
-template <int MODE, bool RZ>
+template <class T, int MODE, bool RZ>
 __global__ __launch_bounds__(256) void k_mg_smooth3(Geo g, Geo gw, Tile3Grid tg,
-                                                    const double* __restrict__ xi,
-                                                    const double* __restrict__ b, Tile3Parent par,
-                                                    double* __restrict__ xo, PcgState* st,
+                                                    const T* __restrict__ xi,
+                                                    const MgRhsT<T, RZ>* __restrict__ b,
+                                                    Tile3Parent par,
+                                                    MgRhsT<T, RZ>* __restrict__ xo, PcgState* st,
                                                     RedBuf rb)
 {
     using namespace tile3;
     if (st->done)
         return;
-    __shared__ double pl[2][TY + 2][prow<1>()];
-    MgSweepEpi<RZ, MODE == 1> epi{gw, b, xo};
-    if constexpr (MODE == 1)
-        tile3_march<double, 1, 1>(gw, tg, b, nullptr, true, 0.0, pl, epi);
-    else
-        tile3_march<double, MODE, 1>(g, tg, xi, nullptr, false, par.alpha, pl, epi, par);
+    __shared__ T pl[2][TY + 2][prow<1>()];
+    MgSweepEpi<T, RZ, MODE == 1> epi{gw, b, xo};
+    if constexpr (MODE == 1) {
+        static_assert(std::is_same<MgRhsT<T, RZ>, T>::value, "the first sweep marches b itself");
+        tile3_march<T, 1, 1>(gw, tg, b, nullptr, true, (T)0, pl, epi);
+    } else {
+        tile3_march<T, MODE, 1>(g, tg, xi, nullptr, false, (T)par.alpha, pl, epi, par);
+    }
     if constexpr (RZ) {
         dd_norm(epi.hi[0], epi.lo[0]);
         grid_reduce<1>(epi.hi, epi.lo, rb, [&](double* sm) {
@@ -215,16 +241,18 @@ __global__ __launch_bounds__(256) void k_mg_smooth3(Geo g, Geo gw, Tile3Grid tg,
 // (children in x, y order), and after the second plane of a z pair (or the
 // last plane) writes the parent.  rotate() runs on every lane of an active
 // block, so the lane exchange is not under divergent control.
+template <class T>
 struct MgRestrictEpi
 {
     const Geo& g;
-    const double* __restrict__ b;
+    const T* __restrict__ b;
     double* __restrict__ bc;
     MgParentMap pm;
     const Tile3Ctx<1>* ctx = nullptr;
-    // b of a plane, alternating with the march's steps
-    double bb[2][tile3::RY], r[tile3::RY] = {0.0, 0.0, 0.0, 0.0};
-    double sum[tile3::RY / 2] = {0.0, 0.0};
+    // b of a plane, alternating with the march's steps; the residuals of
+    // an FP32 level are summed as doubles
+    T bb[2][tile3::RY], r[tile3::RY] = {};
+    double sum[tile3::RY / 2] = {};
     int knext = -1, kcur = -1;
     template <int B>
     __device__ void prefetch(const Tile3Ctx<1>& c, const Plane3Meta& m, int k)
@@ -234,12 +262,12 @@ struct MgRestrictEpi
 #pragma unroll
         for (int q = 0; q < tile3::RY; ++q) {
             const bool ok = k < c.k1 && c.valid[q][0];
-            bb[B][q] = load_any<double>(b, ok, c.colb[q] + m.off, c.safe); // raw: read on valid rows only
+            bb[B][q] = load_any<T>(b, ok, c.colb[q] + m.off, c.safe); // raw: read on valid rows only
         }
     }
     template <int B>
-    __device__ void cells(const Tile3Ctx<1>&, const Plane3Meta&, int q, const double (&)[1],
-                          const double (&y)[1])
+    __device__ void cells(const Tile3Ctx<1>&, const Plane3Meta&, int q, const T (&)[1],
+                          const T (&y)[1])
     {
         r[q] = bb[B][q] - y[0];
     }
@@ -252,7 +280,7 @@ struct MgRestrictEpi
             const bool x1 = gi + 1 < g.n[0];
 #pragma unroll
             for (int p = 0; p < tile3::RY / 2; ++p) {
-                const double a0 = r[2 * p], a1 = r[2 * p + 1];
+                const double a0 = (double)r[2 * p], a1 = (double)r[2 * p + 1];
                 const double b0 = __shfl_down(a0, 1), b1 = __shfl_down(a1, 1);
                 if (even && c.valid[2 * p][0]) {
                     sum[p] = sum[p] + a0;
@@ -281,18 +309,19 @@ struct MgRestrictEpi
     }
 };
 
+template <class T>
 __global__ __launch_bounds__(256) void k_mg_resrestrict3(Geo g, Tile3Grid tg, MgParentMap pm,
-                                                         const double* __restrict__ x,
-                                                         const double* __restrict__ b,
+                                                         const T* __restrict__ x,
+                                                         const T* __restrict__ b,
                                                          double* __restrict__ bc,
                                                          const PcgState* st)
 {
     using namespace tile3;
     if (st->done)
         return;
-    __shared__ double pl[2][TY + 2][prow<1>()];
-    MgRestrictEpi epi{g, b, bc, pm};
-    tile3_march<double, 0, 1>(g, tg, x, nullptr, false, 0.0, pl, epi);
+    __shared__ T pl[2][TY + 2][prow<1>()];
+    MgRestrictEpi<T> epi{g, b, bc, pm};
+    tile3_march<T, 0, 1>(g, tg, x, nullptr, false, (T)0, pl, epi);
 }
 
 // children of parent index P along an axis of n cells
@@ -807,8 +836,9 @@ void launch_mg_smooth3(const Geo& g, const Geo& gw, int src, bool rz, const doub
     const Tile3Grid tg = mg_tile_grid(g);
     const dim3 grid((tg.work + 7) / 8 * 8), blk(kBlock);
     const Tile3Parent par{xc, pm.base, pm.sy, pm.sz, alpha};
-#define CF_MG_SMOOTH3(M, RZ) \
-    hipLaunchKernelGGL((k_mg_smooth3<M, RZ>), grid, blk, 0, s, g, gw, tg, xi, b, par, xo, st, rb)
+#define CF_MG_SMOOTH3(M, RZ)                                                                \
+    hipLaunchKernelGGL((k_mg_smooth3<double, M, RZ>), grid, blk, 0, s, g, gw, tg, xi, b, par, \
+                       xo, st, rb)
     if (src == MG_FIRST) {
         if (rz)
             CF_MG_SMOOTH3(1, true);
@@ -828,12 +858,62 @@ void launch_mg_smooth3(const Geo& g, const Geo& gw, int src, bool rz, const doub
 #undef CF_MG_SMOOTH3
 }
 
+void launch_mg_smooth3_f32(const Geo& g, const Geo& gw, int src, bool rz, const float* xi,
+                           const float* b32, const double* r64, const double* xc,
+                           MgParentMap pm, double alpha, float* xo, double* z64, PcgState* st,
+                           RedBuf rb, hipStream_t s)
+{
+    const Tile3Grid tg = mg_tile_grid(g);
+    const dim3 grid((tg.work + 7) / 8 * 8), blk(kBlock);
+    const Tile3Parent par{xc, pm.base, pm.sy, pm.sz, alpha};
+    if (rz && src == MG_FIRST)
+        throw CfError(CF_EINVAL, "the FP32 first sweep does not reduce r.z");
+#define CF_MG_SMOOTH3F(M, RZ, B, XO)                                                        \
+    hipLaunchKernelGGL((k_mg_smooth3<float, M, RZ>), grid, blk, 0, s, g, gw, tg, xi, B, par, \
+                       XO, st, rb)
+    if (src == MG_FIRST) {
+        CF_MG_SMOOTH3F(1, false, b32, xo);
+    } else if (src == MG_PLAIN) {
+        if (rz)
+            CF_MG_SMOOTH3F(0, true, r64, z64);
+        else
+            CF_MG_SMOOTH3F(0, false, b32, xo);
+    } else {
+        if (rz)
+            CF_MG_SMOOTH3F(2, true, r64, z64);
+        else
+            CF_MG_SMOOTH3F(2, false, b32, xo);
+    }
+#undef CF_MG_SMOOTH3F
+}
+
 void launch_mg_resrestrict3(const Geo& g, const double* x, const double* b, double* bc,
                             MgParentMap pm, const PcgState* st, hipStream_t s)
 {
     const Tile3Grid tg = mg_tile_grid(g);
-    hipLaunchKernelGGL(k_mg_resrestrict3, dim3((tg.work + 7) / 8 * 8), dim3(kBlock), 0, s, g, tg,
-                       pm, x, b, bc, st);
+    hipLaunchKernelGGL(k_mg_resrestrict3<double>, dim3((tg.work + 7) / 8 * 8), dim3(kBlock), 0,
+                       s, g, tg, pm, x, b, bc, st);
+}
+
+void launch_mg_resrestrict3_f32(const Geo& g, const float* x, const float* b, double* bc,
+                                MgParentMap pm, const PcgState* st, hipStream_t s)
+{
+    const Tile3Grid tg = mg_tile_grid(g);
+    hipLaunchKernelGGL(k_mg_resrestrict3<float>, dim3((tg.work + 7) / 8 * 8), dim3(kBlock), 0, s,
+                       g, tg, pm, x, b, bc, st);
+}
+
+void launch_mg0_first_f32(const Geo& g, const MgTab& wd0, const float* b, float* x,
+                          const PcgState* st, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_mg0_first<3, float>), dim3(stream_blocks(g)), dim3(kBlock), 0, s, g,
+                       wd0, b, x, st);
+}
+
+void launch_mg_cast32(const Geo& g, const double* r, float* b32, const PcgState* st,
+                      hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mg_cast32, dim3(stream_blocks(g)), dim3(kBlock), 0, s, g, r, b32, st);
 }
 
 void launch_mg0_resrestrict(const Geo& g, const MgLev& c1, MgParentMap pm, const double* x,

@@ -112,6 +112,22 @@ void launch_mg_smooth3(const Geo& g, const Geo& gw, int src, bool rz, const doub
 void launch_mg_resrestrict3(const Geo& g, const double* x, const double* b, double* bc,
                             MgParentMap pm, const PcgState* st, hipStream_t s);
 
+// FP32 level 0 (3-D, marched): the same sweeps and restriction on FP32
+// vectors.  b32 = (float) r (launch_mg_cast32); a sweep with rz reads r64
+// instead of b32, writes z64 = (double) x and reduces r.z; the restriction
+// sums the FP32 residuals as doubles into the FP64 level 1; the
+// prolongation rounds the FP64 parent to FP32.
+void launch_mg_cast32(const Geo& g, const double* r, float* b32, const PcgState* st,
+                      hipStream_t s);
+void launch_mg0_first_f32(const Geo& g, const MgTab& wd0, const float* b, float* x,
+                          const PcgState* st, hipStream_t s);
+void launch_mg_smooth3_f32(const Geo& g, const Geo& gw, int src, bool rz, const float* xi,
+                           const float* b32, const double* r64, const double* xc,
+                           MgParentMap pm, double alpha, float* xo, double* z64, PcgState* st,
+                           RedBuf rb, hipStream_t s);
+void launch_mg_resrestrict3_f32(const Geo& g, const float* x, const float* b, double* bc,
+                                MgParentMap pm, const PcgState* st, hipStream_t s);
+
 // Compact levels.
 void launch_mg_first(const MgLev& L, const double* b, double* x, const PcgState* st,
                      hipStream_t s);
@@ -137,8 +153,11 @@ class Multigrid
     ~Multigrid();
     // z = B r (one V-cycle), then rho / beta from r.z; returns the level-0
     // buffer that holds z (one of the two the hierarchy owns).  Kernels
-    // return at once when st->done.
-    const double* vcycle(const double* r, PcgState* st, RedBuf rb);
+    // return at once when st->done.  f32: level 0 on FP32 vectors (3-D
+    // grids with more than 2 cells per axis, f32_supported()), the coarser
+    // levels and z in FP64.
+    const double* vcycle(const double* r, PcgState* st, RedBuf rb, bool f32 = false);
+    bool f32_supported() const { return _nt >= 1 && _lev.size() >= 2; }
     int levels() const { return (int)_lev.size(); }
     const MgLev& level(int l) const { return _lev[l]; }
 
@@ -148,6 +167,9 @@ class Multigrid
     void smooth(int l, int src, bool rz, const double* xi, const double* b, double* xo,
                 PcgState* st, RedBuf rb);
     void resrestrict(int l, const double* x, const double* b, PcgState* st);
+    // one FP32 level-0 sweep (launch_mg_smooth3_f32)
+    void smooth32(int src, bool rz, const float* xi, const double* r, float* xo, PcgState* st,
+                  RedBuf rb);
     bool marched(int l) const;
     const Geo& _g;
     hipStream_t _s;

@@ -33,6 +33,7 @@ namespace cfmi
 struct Multigrid::Buffers
 {
     DevArray<double> z[2];
+    DevArray<float> f[3]; // FP32 level 0: x, t, b32 (allocated on first use)
     std::vector<std::unique_ptr<DevArray<double>>> vec;
     std::vector<double*> X, T, B, res;
     MgTail* tail = nullptr; // device copy
@@ -150,8 +151,10 @@ Multigrid::Multigrid(const Geo& g, const cf_config& cfg, hipStream_t s)
     _nt = 0;
     if (last >= 1 && mg_tiled3(g)) {
         Geo gw = g;
-        for (int k = 0; k < 7; ++k)
+        for (int k = 0; k < 7; ++k) {
             gw.dinv[k] = wd0.v[k];
+            gw.dinv32[k] = (float)wd0.v[k]; // the FP32 level 0's w
+        }
         _lg.push_back(g);
         _lgw.push_back(gw);
         _nt = 1;
@@ -273,10 +276,21 @@ void Multigrid::resrestrict(int l, const double* x, const double* b, PcgState* s
         launch_mg_resrestrict(_lev[l], _lev[l + 1], x, b, bc, st, _s);
 }
 
-const double* Multigrid::vcycle(const double* r, PcgState* st, RedBuf rb)
+void Multigrid::smooth32(int src, bool rz, const float* xi, const double* r, float* xo,
+                         PcgState* st, RedBuf rb)
+{
+    Buffers& B = *_buf;
+    launch_mg_smooth3_f32(_lg[0], _lgw[0], src, rz, xi, B.f[2].ptr, r, B.res[1], _pm[0],
+                          _lev[0].alpha, xo, B.z[0].ptr, st, rb, _s);
+}
+
+const double* Multigrid::vcycle(const double* r, PcgState* st, RedBuf rb, bool f32)
 {
     Buffers& B = *_buf;
     const int last = (int)_lev.size() - 1;
+    if (f32 && !f32_supported())
+        throw CfError(CF_ENOTIMPL, "the FP32 multigrid V-cycle needs a 3-D grid with more "
+                                   "than 2 cells along every axis");
     if (last == 0) {
         // level 0 is the coarsest: coarse sweeps from zero
         double* x = B.X[0];
@@ -295,8 +309,29 @@ const double* Multigrid::vcycle(const double* r, PcgState* st, RedBuf rb)
         launch_mgpcg_rz(_g, r, x, st, rb, _s);
         return x;
     }
+    // FP32 level 0: b32 = (float) r, pre-smoothing and restriction into the
+    // FP64 level 1 (the up phase below ends level 0 in FP32 too)
+    const int l0 = f32 ? 1 : 0; // first level of the FP64 loops
+    float* x32 = nullptr;
+    if (f32) {
+        for (int i = 0; i < 3; ++i)
+            if (!B.f[i].ptr)
+                B.f[i].alloc(_g.total, _s);
+        float *x = B.f[0].ptr, *t = B.f[1].ptr;
+        launch_mg_cast32(_g, r, B.f[2].ptr, st, _s);
+        if (_nu == 1)
+            launch_mg0_first_f32(_g, _wt[0], B.f[2].ptr, x, st, _s);
+        else
+            smooth32(MG_FIRST, false, nullptr, r, x, st, rb);
+        for (int it = 2; it < _nu; ++it) {
+            smooth32(MG_PLAIN, false, x, r, t, st, rb);
+            std::swap(x, t);
+        }
+        launch_mg_resrestrict3_f32(_lg[0], x, B.f[2].ptr, B.B[1], _pm[0], st, _s);
+        x32 = x;
+    }
     // down: pre-smoothing from zero and restriction
-    for (int l = 0; l < _tail0; ++l) {
+    for (int l = l0; l < _tail0; ++l) {
         const double* b = l == 0 ? r : B.B[l];
         double* x = B.X[l];
         double* t = B.T[l];
@@ -315,7 +350,7 @@ const double* Multigrid::vcycle(const double* r, PcgState* st, RedBuf rb)
     // last sweep also reduces r.z
     const bool pre_odd = (_nu >= 2 ? _nu - 2 : 0) & 1;
     double* x = nullptr;
-    for (int l = _tail0 - 1; l >= 0; --l) {
+    for (int l = _tail0 - 1; l >= l0; --l) {
         const double* b = l == 0 ? r : B.B[l];
         x = pre_odd ? B.T[l] : B.X[l];
         double* t = pre_odd ? B.X[l] : B.T[l];
@@ -325,6 +360,19 @@ const double* Multigrid::vcycle(const double* r, PcgState* st, RedBuf rb)
             smooth(l, MG_PLAIN, l == 0 && it == _nu - 1, x, b, t, st, rb);
             std::swap(x, t);
         }
+    }
+    if (f32) {
+        // level 0 in FP32: the last sweep writes z = (double) x to z[0] and
+        // reduces r.z from the FP64 r
+        float* xf = x32;
+        float* tf = xf == B.f[0].ptr ? B.f[1].ptr : B.f[0].ptr;
+        smooth32(MG_PROLONG, _nu == 1, xf, r, tf, st, rb);
+        std::swap(xf, tf);
+        for (int it = 1; it < _nu; ++it) {
+            smooth32(MG_PLAIN, it == _nu - 1, xf, r, tf, st, rb);
+            std::swap(xf, tf);
+        }
+        return B.z[0].ptr;
     }
     return x;
 }

@@ -490,6 +490,8 @@ cf_status VelocityCorrector::solveRefined(const double* b, double* x,
 cf_status VelocityCorrector::solve(int precision, const double* b, double* x,
                                    cf_solve_stats* st)
 {
+    if (precision == CF_FP32_REFINED && _cfg.preconditioner == CF_PRECOND_MG)
+        return solveMg(b, x, st, true);
     if (precision == CF_FP32_REFINED) {
         const cf_status rc = solveRefined(b, x, st);
         zeroMeanIfUnpreconditioned(x);
@@ -498,7 +500,7 @@ cf_status VelocityCorrector::solve(int precision, const double* b, double* x,
     if (precision != CF_FP64)
         throw CfError(CF_EINVAL, "precision must be CF_FP64 or CF_FP32_REFINED");
     if (_cfg.preconditioner == CF_PRECOND_MG)
-        return solveMg(b, x, st);
+        return solveMg(b, x, st, false);
     const cf_status rc = solve64(b, x, st);
     zeroMeanIfUnpreconditioned(x);
     return rc;
@@ -517,19 +519,25 @@ void VelocityCorrector::zeroMeanIfUnpreconditioned(double* x)
     launch_rhs_shift(_g, x, _scratch.ptr + 1, _s);
 }
 
-// MG-PCG: Jacobi-PCG's kernels with z = B r from one V-cycle.  The
+// MG-PCG: Jacobi-PCG's kernels with z = B r from one V-cycle (with f32, a
+// V-cycle whose level 0 runs on FP32 vectors; every CG vector, dot and
+// scalar stays FP64, so the solve reaches the FP64 tolerance).  The
 // direction kernel runs on a geometry whose Jacobi table is 1, so it forms
 // p = fma(beta, p_old, z) (z alone on the first iteration); the update
 // kernel leaves beta and rho to the V-cycle's last sweep, which reduces r.z.
 // Iterations are queued in batches like runIterations; every kernel of an
 // iteration queued after convergence returns at once.
-cf_status VelocityCorrector::solveMg(const double* b, double* x, cf_solve_stats* st)
+cf_status VelocityCorrector::solveMg(const double* b, double* x, cf_solve_stats* st,
+                                     bool f32)
 {
     if (!_mg)
         _mg.reset(new Multigrid(_g, _cfg, _s));
+    if (f32 && !_mg->f32_supported())
+        throw CfError(CF_ENOTIMPL, "the FP32 multigrid V-cycle needs a 3-D grid with more "
+                                   "than 2 cells along every axis");
     RedBuf rb{_red_partials.ptr, _red_counter.ptr};
     launch_mgpcg_init(_g, b, x, _r.ptr, _st, _cfg.cg_tol, _cfg.cg_max_iter, rb, _s);
-    const double* z = _mg->vcycle(_r.ptr, _st, rb);
+    const double* z = _mg->vcycle(_r.ptr, _st, rb, f32);
     PcgState hs;
     readState(hs);
     int cur = 0, launched = 0;
@@ -548,7 +556,7 @@ cf_status VelocityCorrector::solveMg(const double* b, double* x, cf_solve_stats*
             _timer.end(t1, _s, TK_UPDATE, it);
             // the V-cycle prepares iteration it + 1 (a no-op after the stop)
             int t2 = _timer.begin_iter(_s, it + 1);
-            z = _mg->vcycle(_r.ptr, _st, rb);
+            z = _mg->vcycle(_r.ptr, _st, rb, f32);
             _timer.end(t2, _s, TK_VCYCLE, it + 1);
             cur ^= 1;
         }

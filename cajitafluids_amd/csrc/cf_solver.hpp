@@ -179,7 +179,9 @@ class VelocityCorrector
     cf_status solveRefined(const double* b, double* x, cf_solve_stats* st);
     cf_status solveRestart(const double* b, double* x, cf_solve_stats& loc);
     cf_status solveCarry(const double* b, double* x, cf_solve_stats& loc);
-    cf_status solveMg(const double* b, double* x, cf_solve_stats* st);
+    // f32: the V-cycle's level 0 in FP32 (CF_FP32_REFINED with the multigrid
+    // preconditioner); the CG around it stays FP64
+    cf_status solveMg(const double* b, double* x, cf_solve_stats* st, bool f32);
     template <class T>
     int runIterations(T* x, T* r, DevArray<T> (&p)[2], int& cur, int predict,
                       int tk0);

@@ -423,7 +423,8 @@ __device__ __forceinline__ void tile3_load(const Tile3Ctx<CPL>& c, const Plane3M
             const int64_t pat = par.base + (c.sgi[s] >> 1) + par.sy * (c.sgj[s] >> 1) +
                                 par.sz * (m.zc >> 1);
             P.a[s][0] = load_any<T>(va, ok, at, c.safe);
-            P.b[s][0] = load_any<T>(par.xc, ok, pat, par.base);
+            // the parent level is FP64; an FP32 level rounds its value
+            P.b[s][0] = (T)load_any<double>(par.xc, ok, pat, par.base);
             continue;
         } else if constexpr (CPL == 1) {
             // the direction's old value is read on the first iteration too

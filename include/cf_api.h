@@ -91,9 +91,12 @@ typedef struct cf_config {
                              replaced by the FP64 one at each pass) */
     /* Pressure-solve preconditioner (VelocityCorrector.hpp:300-339 names
      * the choice; the reference's default is HYPRE's PCG + PFMG): Jacobi,
-     * a geometric multigrid V-cycle (FP64 only), or none ("None",
-     * VelocityCorrector.hpp:327: plain CG).  With MG or none, the solve
-     * returns the zero-mean pressure on a singular operator. */
+     * a geometric multigrid V-cycle, or none ("None",
+     * VelocityCorrector.hpp:327: plain CG).  With MG, CF_FP32_REFINED runs
+     * the V-cycle's finest level on FP32 vectors inside the FP64 CG (3-D
+     * grids with more than 2 cells along every axis; CF_ENOTIMPL otherwise).
+     * With MG or none, the solve returns the zero-mean pressure on a
+     * singular operator. */
     int preconditioner;   /* CF_PRECOND_JACOBI, CF_PRECOND_MG or CF_PRECOND_NONE */
     int mg_nu;            /* Jacobi sweeps before and after each coarse
                              correction (>= 1) */

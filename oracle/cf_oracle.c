@@ -762,7 +762,7 @@ int cfo_pcg64(struct cfo_state* s, const double* b, double* x, double tol,
  *           (alpha32 = (float)alpha, beta32 = (float)beta)
  *         x = x + (double) d; r64 = b - A x
  */
-static void cfo_spmv32(struct cfo_state* s, const float* x, float* y)
+void cfo_spmv32(struct cfo_state* s, const float* x, float* y)
 {
     int lo[3], hi[3];
     cfo_owned_range(s, 0, lo, hi);
@@ -789,7 +789,7 @@ static void cfo_spmv32(struct cfo_state* s, const float* x, float* y)
             }
 }
 
-static void cfo_fill_ghosts_f32(struct cfo_state* s, float* a)
+void cfo_fill_ghosts_f32(struct cfo_state* s, float* a)
 {
     const int* E = s->ext[0];
     for (int d = 0; d < s->dim; ++d) {
@@ -1005,6 +1005,8 @@ int cfo_pressure_solve(cfo_state* s, int precision, cfo_stats* st)
     memset(st, 0, sizeof(*st));
     /* VelocityCorrector.hpp:272: lhs = 0 before every solve (no warm start). */
     CFO_FOR_OWNED_CELLS(s, s->lhs[c] = 0.0);
+    if (precision == CFO_FP32_REFINED && s->c.preconditioner == CFO_PRECOND_MG)
+        return cfo_pcg_mg(s, s->rhs, s->lhs, s->c.cg_tol, s->c.cg_max_iter, st, 1);
     if (precision == CFO_FP32_REFINED) {
         if (!s->x32) {
             long nc = s->size[0];
@@ -1034,7 +1036,7 @@ int cfo_solve64(struct cfo_state* s, const double* b, double* x, double tol,
                 int maxit, cfo_stats* st)
 {
     if (s->c.preconditioner == CFO_PRECOND_MG)
-        return cfo_pcg_mg(s, b, x, tol, maxit, st);
+        return cfo_pcg_mg(s, b, x, tol, maxit, st, 0);
     int rc = cfo_pcg64(s, b, x, tol, maxit, st);
     if (s->c.preconditioner == CFO_PRECOND_NONE && cfo_operator_singular(s))
         cfo_remove_mean(s, x);

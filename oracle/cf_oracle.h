@@ -54,6 +54,7 @@ enum { CFO_FP64 = 0, CFO_FP32_REFINED = 1 };
 #define CFO_EINVAL 1
 #define CFO_ENOMEM 2
 #define CFO_ENOCONV 3
+#define CFO_ENOTIMPL 5 /* combination not restated (the library returns CF_ENOTIMPL too) */
 
 /* Same field order and meaning as the product's cf_config. */
 typedef struct cfo_config {
@@ -159,6 +160,8 @@ int cfo_mg_nlevels(cfo_state* s);
 int cfo_mg_level_dims(cfo_state* s, int level, int n[3]);
 int cfo_mg_apply_level(cfo_state* s, int level, const double* x, double* y);
 int cfo_mg_vcycle(cfo_state* s, const double* r, double* z);
+/* the V-cycle with level 0 in FP32 (f32 = 1) or FP64 (0) */
+int cfo_mg_vcycle_prec(cfo_state* s, int f32, const double* r, double* z);
 
 /* Adjoint (reverse mode) of setup + nsteps, checkpointing every ckpt steps.
  * Inputs: the initial state held in Current, the loss weights dL/dq_T in

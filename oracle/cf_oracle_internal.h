@@ -84,8 +84,12 @@ int cfo_in_source(const struct cfo_state* s, const double x[3]);
 void cfo_spmv(struct cfo_state* s, const double* x, double* y);
 int cfo_pcg64(struct cfo_state* s, const double* b, double* x, double tol,
               int maxit, cfo_stats* st);
+/* MG-PCG; f32: the V-cycle's level 0 in FP32 (cf_oracle_mg.c) */
 int cfo_pcg_mg(struct cfo_state* s, const double* b, double* x, double tol,
-               int maxit, cfo_stats* st);
+               int maxit, cfo_stats* st, int f32);
+/* FP32 stored-stencil SpMV and periodic ghost fill on level 0 */
+void cfo_spmv32(struct cfo_state* s, const float* x, float* y);
+void cfo_fill_ghosts_f32(struct cfo_state* s, float* a);
 /* the FP64 solve of the configured preconditioner */
 int cfo_solve64(struct cfo_state* s, const double* b, double* x, double tol,
                 int maxit, cfo_stats* st);

@@ -30,6 +30,16 @@
  *              (Braess's remedy for the piecewise-constant P, whose coarse
  *              correction alone is too small by about half in energy)
  *   MG-PCG     as Jacobi-PCG with z = V(r) (cfo_pcg_mg)
+ *   FP32 level 0  (precision CF_FP32_REFINED, 3-D, every axis > 2 cells):
+ *              the V-cycle's level 0 runs on FP32 vectors with the FP32
+ *              stencil (coefficients rounded from the FP64 ones, w0 rounded
+ *              from omega * M): b32 = (float) r; x = w0 b32, then nu - 1
+ *              sweeps fmaf(w0, b32 - A32 x, x); the residual b32 - A32 x in
+ *              FP32, restricted by summing the children as doubles (z, y, x
+ *              order) into level 1, which stays FP64; prolongation
+ *              fmaf((float) alpha, (float) x_parent, x); nu sweeps;
+ *              z = (double) x.  The CG around it (residuals, dots, alpha,
+ *              beta, x) stays FP64.
  *
  * A-x order everywhere: diag x, then fma over -x, +x, -y, +y, -z, +z.
  */
@@ -56,6 +66,8 @@ struct cfo_mg {
     double* r0;        /* level-0 residual (ghosted) */
     double* t0;        /* level-0 Jacobi buffer (ghosted) */
     double* w0;        /* level-0 omega * M (ghosted) */
+    /* FP32 level 0 (ghosted, allocated on first use) */
+    float *x0f, *t0f, *b0f, *r0f, *w0f;
 };
 
 static long lidx(const cfo_mglev* L, int i, int j, int k)
@@ -98,6 +110,11 @@ void cfo_mg_free(struct cfo_state* s)
     free(mg->r0);
     free(mg->t0);
     free(mg->w0);
+    free(mg->x0f);
+    free(mg->t0f);
+    free(mg->b0f);
+    free(mg->r0f);
+    free(mg->w0f);
     free(mg);
     s->mg = NULL;
 }
@@ -362,14 +379,123 @@ static void mg_vcycle_level(struct cfo_state* s, int l, double* x0, const double
     }
 }
 
+/* ---- FP32 level 0 ------------------------------------------------------------------ */
+
+/* The FP32 level 0 runs where the device runs it: 3-D, every axis coarsened
+ * at least once (more than 2 cells), so that level 0 is a marched level. */
+static int mg_f32_supported(const struct cfo_state* s)
+{
+    return s->dim == 3 && s->n[0] > 2 && s->n[1] > 2 && s->n[2] > 2 && s->mg->nlev >= 2;
+}
+
+static int mg_f32_alloc(struct cfo_state* s)
+{
+    struct cfo_mg* mg = s->mg;
+    if (mg->x0f)
+        return CFO_OK;
+    long nc = s->size[0];
+    mg->x0f = (float*)calloc((size_t)nc, sizeof(float));
+    mg->t0f = (float*)calloc((size_t)nc, sizeof(float));
+    mg->b0f = (float*)calloc((size_t)nc, sizeof(float));
+    mg->r0f = (float*)calloc((size_t)nc, sizeof(float));
+    mg->w0f = (float*)calloc((size_t)nc, sizeof(float));
+    if (!mg->x0f || !mg->t0f || !mg->b0f || !mg->r0f || !mg->w0f)
+        return CFO_ENOMEM;
+    CFO_FOR_OWNED_CELLS(s, mg->w0f[c] = (float)mg->w0[c]);
+    return CFO_OK;
+}
+
+/* level-0 FP32 sweep on ghosted arrays: xo = fmaf(w0, b - A32 xi, xi) */
+static void mg_sweep0_f32(struct cfo_state* s, float* xi, const float* b, float* xo)
+{
+    struct cfo_mg* mg = s->mg;
+    if (s->c.periodic[0] || s->c.periodic[1] || s->c.periodic[2])
+        cfo_fill_ghosts_f32(s, xi);
+    cfo_spmv32(s, xi, xo);
+    CFO_FOR_OWNED_CELLS(s, xo[c] = fmaf(mg->w0f[c], b[c] - xo[c], xi[c]));
+}
+
+/* z = B r with level 0 in FP32 (levels >= 1 as mg_vcycle_level) */
+static void mg_vcycle0_f32(struct cfo_state* s, double* z, const double* r)
+{
+    struct cfo_mg* mg = s->mg;
+    float *x = mg->x0f, *t = mg->t0f, *b = mg->b0f, *rf = mg->r0f;
+    const float* w = mg->w0f;
+    const int nu = s->c.mg_nu;
+    const int periodic = s->c.periodic[0] || s->c.periodic[1] || s->c.periodic[2];
+    CFO_FOR_OWNED_CELLS(s, b[c] = (float)r[c]; x[c] = w[c] * b[c]);
+    for (int it = 1; it < nu; ++it) {
+        mg_sweep0_f32(s, x, b, t);
+        CFO_FOR_OWNED_CELLS(s, x[c] = t[c]);
+    }
+    if (periodic)
+        cfo_fill_ghosts_f32(s, x);
+    cfo_spmv32(s, x, rf);
+    CFO_FOR_OWNED_CELLS(s, rf[c] = b[c] - rf[c]);
+    /* restriction: the children's FP32 residuals summed as doubles */
+    {
+        const cfo_mglev* F = &mg->lev[0];
+        const cfo_mglev* C = &mg->lev[1];
+        for (int K = 0; K < C->n[2]; ++K)
+            for (int J = 0; J < C->n[1]; ++J)
+                for (int I = 0; I < C->n[0]; ++I) {
+                    int Ic[3] = {I, J, K}, lo[3], hi[3];
+                    for (int d = 0; d < 3; ++d) {
+                        lo[d] = 2 * Ic[d];
+                        hi[d] = 2 * Ic[d] + 1 < F->n[d] ? 2 * Ic[d] + 1 : 2 * Ic[d];
+                    }
+                    double sum = 0.0;
+                    for (int k = lo[2]; k <= hi[2]; ++k)
+                        for (int j = lo[1]; j <= hi[1]; ++j)
+                            for (int i = lo[0]; i <= hi[0]; ++i)
+                                sum = sum + (double)rf[g0(s, i, j, k)];
+                    mg->lev[1].b[lidx(C, I, J, K)] = sum;
+                }
+    }
+    mg_vcycle_level(s, 1, NULL, NULL);
+    /* prolongation in FP32 from the FP64 level-1 result */
+    {
+        const cfo_mglev* F = &mg->lev[0];
+        const cfo_mglev* C = &mg->lev[1];
+        const float a = (float)s->c.mg_alpha;
+        for (int k = 0; k < F->n[2]; ++k)
+            for (int j = 0; j < F->n[1]; ++j)
+                for (int i = 0; i < F->n[0]; ++i) {
+                    long fi = g0(s, i, j, k);
+                    x[fi] = fmaf(a, (float)mg->lev[1].x[lidx(C, i / 2, j / 2, k / 2)], x[fi]);
+                }
+    }
+    for (int it = 0; it < nu; ++it) {
+        mg_sweep0_f32(s, x, b, t);
+        CFO_FOR_OWNED_CELLS(s, x[c] = t[c]);
+    }
+    CFO_FOR_OWNED_CELLS(s, z[c] = (double)x[c]);
+}
+
+/* the preconditioner of MG-PCG: z = B r */
+static void mg_precondition(struct cfo_state* s, int f32, double* z, const double* r)
+{
+    if (f32)
+        mg_vcycle0_f32(s, z, r);
+    else
+        mg_vcycle_level(s, 0, z, r);
+}
+
 /* ---- MG-PCG ---------------------------------------------------------------------- */
 
 int cfo_pcg_mg(struct cfo_state* s, const double* b, double* x, double tol, int maxit,
-               cfo_stats* st)
+               cfo_stats* st, int f32)
 {
     int rc = cfo_mg_build(s);
     if (rc != CFO_OK)
         return rc;
+    if (f32) {
+        if (!mg_f32_supported(s))
+            return CFO_ENOTIMPL;
+        rc = mg_f32_alloc(s);
+        if (rc != CFO_OK)
+            return rc;
+    }
     double *r = s->r, *z = s->z, *p = s->p, *q = s->q;
     int periodic = s->c.periodic[0] || s->c.periodic[1] || s->c.periodic[2];
     double bnorm = sqrt(cfo_dot_dd(s, b, b));
@@ -387,7 +513,7 @@ int cfo_pcg_mg(struct cfo_state* s, const double* b, double* x, double tol, int 
         return CFO_OK;
     }
     CFO_FOR_OWNED_CELLS(s, r[c] = b[c]);
-    mg_vcycle_level(s, 0, z, r);
+    mg_precondition(s, f32, z, r);
     CFO_FOR_OWNED_CELLS(s, p[c] = z[c]);
     double rho = cfo_dot_dd(s, r, z);
     double res = 1.0;
@@ -407,7 +533,7 @@ int cfo_pcg_mg(struct cfo_state* s, const double* b, double* x, double tol, int 
             conv = 1;
             break;
         }
-        mg_vcycle_level(s, 0, z, r);
+        mg_precondition(s, f32, z, r);
         double rho_new = cfo_dot_dd(s, r, z);
         double beta = rho_new / rho;
         CFO_FOR_OWNED_CELLS(s, p[c] = fma(beta, p[c], z[c]));
@@ -496,8 +622,19 @@ int cfo_mg_apply_level(cfo_state* s, int level, const double* x, double* y)
 
 int cfo_mg_vcycle(cfo_state* s, const double* r, double* z)
 {
+    return cfo_mg_vcycle_prec(s, 0, r, z);
+}
+
+int cfo_mg_vcycle_prec(cfo_state* s, int f32, const double* r, double* z)
+{
     if (cfo_mg_build(s) != CFO_OK)
         return CFO_ENOMEM;
+    if (f32) {
+        if (!mg_f32_supported(s))
+            return CFO_ENOTIMPL;
+        if (mg_f32_alloc(s) != CFO_OK)
+            return CFO_ENOMEM;
+    }
     double* gr = (double*)calloc((size_t)s->size[0], sizeof(double));
     double* gz = (double*)calloc((size_t)s->size[0], sizeof(double));
     if (!gr || !gz) {
@@ -506,7 +643,7 @@ int cfo_mg_vcycle(cfo_state* s, const double* r, double* z)
         return CFO_ENOMEM;
     }
     to_ghosted(s, r, gr);
-    mg_vcycle_level(s, 0, gz, gr);
+    mg_precondition(s, f32, gz, gr);
     from_ghosted(s, gz, z);
     free(gr);
     free(gz);

@@ -151,6 +151,7 @@ def lib():
         "cfo_mg_level_dims": (C.c_int, [P, C.c_int, C.POINTER(C.c_int)]),
         "cfo_mg_apply_level": (C.c_int, [P, C.c_int, dp, dp]),
         "cfo_mg_vcycle": (C.c_int, [P, dp, dp]),
+        "cfo_mg_vcycle_prec": (C.c_int, [P, C.c_int, dp, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -341,10 +342,11 @@ class Oracle:
             raise ValueError(f"cfo_mg_apply_level rc={rc}")
         return y
 
-    def mg_vcycle(self, r):
+    def mg_vcycle(self, r, f32=False):
+        """z = B r on compact level-0 vectors; f32: level 0 in FP32."""
         r = np.ascontiguousarray(r, dtype=np.float64).ravel()
         z = np.zeros_like(r)
-        rc = self.L.cfo_mg_vcycle(self.h, _dp(r), _dp(z))
+        rc = self.L.cfo_mg_vcycle_prec(self.h, 1 if f32 else 0, _dp(r), _dp(z))
         if rc != 0:
             raise ValueError(f"cfo_mg_vcycle rc={rc}")
         return z

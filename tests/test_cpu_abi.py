@@ -110,12 +110,18 @@ def test_factory_maps_hypre_preconditioner_names():
                           solver, "PFMG")
 
 
-def test_multigrid_fp32_not_implemented():
-    cfg = K.base_config(2, 8)
+def test_multigrid_fp32_needs_3d():
+    """The FP32 V-cycle runs on 3-D grids with more than 2 cells along every
+    axis (level 0 marched by the tiles); elsewhere CF_ENOTIMPL."""
+    for dim, n in ((2, 8), (3, 2)):
+        cfg = K.base_config(dim, n)
+        cfg.update(preconditioner=K.PRECOND_MG, precision=K.FP32_REFINED)
+        with pytest.raises(_lib.CfError) as ei:
+            layout(cfg, K.CELL)
+        assert ei.value.code == _lib.CF_ENOTIMPL
+    cfg = K.base_config(3, 8)
     cfg.update(preconditioner=K.PRECOND_MG, precision=K.FP32_REFINED)
-    with pytest.raises(_lib.CfError) as ei:
-        layout(cfg, K.CELL)
-    assert ei.value.code == _lib.CF_ENOTIMPL
+    layout(cfg, K.CELL)
 
 
 def test_create_without_device_fails_cleanly():

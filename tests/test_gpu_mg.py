@@ -6,7 +6,9 @@ same fma() calls and the same coarse coefficients (integer face counts times
 dt / (rho dx^2), rounded once), and the PCG dots are double-double, so the
 iterates agree to rounding and the iteration counts are equal.  Truncated
 solves (1, 2 and 5 iterations) compare x after the first V-cycles, where a
-difference in one kernel would show before convergence hides it.
+difference in one kernel would show before convergence hides it.  The same
+holds with precision CF_FP32_REFINED, where level 0 of the V-cycle runs on
+FP32 vectors (3-D) on both sides.
 """
 import numpy as np
 import pytest
@@ -207,3 +209,75 @@ def test_mg_adjoint_matches_oracle(cf, oracle_mod, name):
         assert rel_l2(g[k][sl], go[k][sl]) <= 1e-10, k
     orc.close()
 
+
+
+F32_CASES = [n for n in sorted(CASES) if CASES[n]["dim"] == 3]
+
+
+@pytest.mark.parametrize("name", F32_CASES)
+def test_mg_pcg_fp32_level0_matches_oracle(cf, oracle_mod, name):
+    """FP32 level 0 inside the FP64 CG: identical iteration counts and x
+    against the oracle's FP32-level-0 MG-PCG, final residual below the FP64
+    tolerance, and x differs from the FP64 MG solve's (the FP32 V-cycle ran)."""
+    cfg = dict(CASES[name], precision=K.FP32_REFINED)
+    print(f"mg-pcg fp32 level 0 vs oracle: {name} {cfg['num_cell']}", flush=True)
+    sol, orc = _prepare(cf, oracle_mod, cfg)
+    st_g = sol.pressure_solve(K.FP32_REFINED)
+    rc, st_o = orc.pressure_solve(K.FP32_REFINED)
+    assert rc == 0 and st_g["converged"]
+    assert st_g["iterations"] == st_o["iterations"]
+    assert st_g["final_residual"] < cfg["cg_tol"]
+    x_g = sol.owned_view(sol.get_pressure(0), 0).copy()
+    x_o = orc.owned_view(orc.lhs(), 0)
+    assert rel_l2(x_g, x_o) <= TOL
+    sol.pressure_solve(K.FP64)
+    x64 = sol.owned_view(sol.get_pressure(0), 0)
+    assert not np.array_equal(x_g, x64)
+    sol.close()
+    orc.close()
+
+
+@pytest.mark.parametrize("maxit", [1, 2, 5])
+@pytest.mark.parametrize("name", ["3d_ragged_70x36x40", "3d_periodic_xy_21x18x11",
+                                  "3d_nu1_coarse4_20", "3d_odd_45", "3d_periodic_x_132"])
+def test_mg_pcg_fp32_level0_truncated(cf, oracle_mod, name, maxit):
+    from cajitafluids_amd import _lib
+    cfg = dict(CASES[name], precision=K.FP32_REFINED, cg_max_iter=maxit)
+    print(f"mg-pcg fp32 level 0 truncated: {name} maxit={maxit}", flush=True)
+    sol, orc = _prepare(cf, oracle_mod, cfg, seed=11)
+    with pytest.raises(_lib.CfError) as ei:
+        sol.pressure_solve(K.FP32_REFINED)
+    assert ei.value.code == _lib.CF_ENOCONV
+    rc, st_o = orc.pressure_solve(K.FP32_REFINED)
+    assert rc == _lib.CF_ENOCONV
+    assert sol.last_stats["iterations"] == st_o["iterations"] == maxit
+    x_g = sol.owned_view(sol.get_pressure(0), 0)
+    x_o = orc.owned_view(orc.lhs(), 0)
+    assert rel_l2(x_g, x_o) <= TOL
+    assert abs(sol.last_stats["final_residual"] - st_o["final_residual"]) \
+        <= 1e-12 * st_o["final_residual"]
+    sol.close()
+    orc.close()
+
+
+def test_mg_fp32_level0_full_steps(cf, oracle_mod):
+    """Setup and 4 steps of the 24^3 vortex tank with the FP32-level-0
+    V-cycle, against the oracle's."""
+    cfg = K.vortex_tank_3d(24)
+    cfg.update(preconditioner=K.PRECOND_MG, precision=K.FP32_REFINED)
+    sol = cf.Solver(cfg)
+    orc = oracle_mod.Oracle(cfg)
+    sol.init_vortex_tubes(1)
+    orc.init_vortex_tubes(1)
+    st_g = sol.setup()
+    rc, st_o = orc.setup()
+    assert rc == 0 and st_g["iterations"] == st_o["iterations"]
+    for n in range(4):
+        st_g = sol.step()
+        rc, st_o = orc.step()
+        assert rc == 0 and st_g["iterations"] == st_o["iterations"], n
+        assert st_g["final_residual"] < cfg["cg_tol"]
+    for e, (rl2, mx) in compare_state(sol, orc).items():
+        assert rl2 <= TOL, f"entity {e}: rel L2 {rl2:.3e} (max abs {mx:.3e})"
+    sol.close()
+    orc.close()

@@ -8,6 +8,9 @@
 - A dense-matrix V-cycle gives the same z as the stencil V-cycle: damped
   Jacobi as D^-1 matrices, restriction P^T, prolongation P.
 - MG-PCG reaches A x = b with x equal to pinv(A) b up to the null space.
+- With level 0 in FP32 (3-D), the V-cycle equals the FP64 one to FP32
+  rounding (and differs from it: the flag is not ignored), and MG-PCG still
+  reaches pinv(A) b; in 2-D the combination is not provided.
 """
 import numpy as np
 import pytest
@@ -228,4 +231,54 @@ def test_coarse_operators_closed_form(oracle_mod, name):
     for l in range(1, nlev):
         A = _dense(lambda x, l=l: o.mg_apply(l, x), int(np.prod(dims[l])))
         np.testing.assert_array_equal(A, _closed_form_operator(cfg, dims, l, s))
+    o.close()
+
+
+F32_CASES = ["3d_solid_6x5x4", "3d_periodic_xy_5x6x3"]
+
+
+@pytest.mark.parametrize("name", F32_CASES)
+def test_fp32_level0_vcycle_is_the_fp64_one_rounded(oracle_mod, name):
+    cfg = CASES[name]
+    o = oracle_mod.Oracle(cfg)
+    N = int(np.prod(cfg["num_cell"]))
+    rng = np.random.default_rng(2)
+    for _ in range(3):
+        r = rng.normal(size=N)
+        z64 = o.mg_vcycle(r)
+        z32 = o.mg_vcycle(r, f32=True)
+        d = np.linalg.norm(z32 - z64) / np.linalg.norm(z64)
+        assert 1e-10 < d < 1e-5, d
+    o.close()
+
+
+@pytest.mark.parametrize("name", F32_CASES)
+def test_fp32_mg_pcg_solves_to_pinv(oracle_mod, name):
+    cfg = dict(CASES[name])
+    cfg["cg_tol"] = 1e-12
+    cfg["precision"] = K.FP32_REFINED
+    o = oracle_mod.Oracle(cfg)
+    o.init_random_faces(3)
+    o.build_rhs()
+    o.rhs_remove_mean()
+    b = o.owned_view(o.rhs(), 0).reshape(-1).copy()
+    rc, st = o.pressure_solve(oracle_mod.FP32_REFINED)
+    assert rc == 0 and st["converged"]
+    x = o.owned_view(o.lhs(), 0).reshape(-1).copy()
+    A = _dense(lambda v: o.mg_apply(0, v), x.size)
+    xp = np.linalg.pinv(A) @ b
+    assert np.linalg.norm(x - xp) <= 1e-9 * np.linalg.norm(xp)
+    assert np.linalg.norm(A @ x - b) <= 1e-11 * np.linalg.norm(b)
+    o.close()
+
+
+def test_fp32_mg_is_3d_only(oracle_mod):
+    cfg = dict(CASES["2d_solid_7x5"], precision=K.FP32_REFINED)
+    o = oracle_mod.Oracle(cfg)
+    o.init_random_faces(3)
+    o.build_rhs()
+    rc, _ = o.pressure_solve(oracle_mod.FP32_REFINED)
+    assert rc == 5  # CF_ENOTIMPL
+    with pytest.raises(ValueError):
+        o.mg_vcycle(np.ones(35), f32=True)
     o.close()
