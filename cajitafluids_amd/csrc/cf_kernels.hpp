@@ -97,10 +97,12 @@ void launch_pcg_dir_spmv(const Geo& g, const T* r, const T* p_old, T* p_new,
 // iteration k read and left intact.  A run that stops after an X_SKIP
 // update completes x with launch_pcg_xfix.
 enum XUpdate { X_EACH = 0, X_SKIP = 1, X_PAIR = 2 };
+// r32 (3-D FP64 only): also write (float) r there, the right-hand side of
+// an FP32 finest multigrid level.
 template <class T>
 void launch_pcg_update(const Geo& g, T* x, T* r, const T* p, PcgState* st,
                        RedBuf rb, hipStream_t s, int xmode = X_EACH,
-                       const T* p_lag = nullptr);
+                       const T* p_lag = nullptr, float* r32 = nullptr);
 // x = fma(alpha, p, x) over the owned cells with the state's alpha: the x
 // step of a run's last iteration when that was an X_SKIP update.
 template <class T>

@@ -156,8 +156,13 @@ class Multigrid
     // return at once when st->done.  f32: level 0 on FP32 vectors (3-D
     // grids with more than 2 cells per axis, f32_supported()), the coarser
     // levels and z in FP64.
-    const double* vcycle(const double* r, PcgState* st, RedBuf rb, bool f32 = false);
+    // b32_ready: the caller has written (float) r into b32() (the PCG
+    // update does), so the FP32 V-cycle skips its own cast.
+    const double* vcycle(const double* r, PcgState* st, RedBuf rb, bool f32 = false,
+                         bool b32_ready = false);
     bool f32_supported() const { return _nt >= 1 && _lev.size() >= 2; }
+    // the FP32 level 0's right-hand side (null before the first FP32 V-cycle)
+    float* b32() const;
     int levels() const { return (int)_lev.size(); }
     const MgLev& level(int l) const { return _lev[l]; }
 

@@ -284,7 +284,10 @@ void Multigrid::smooth32(int src, bool rz, const float* xi, const double* r, flo
                           _lev[0].alpha, xo, B.z[0].ptr, st, rb, _s);
 }
 
-const double* Multigrid::vcycle(const double* r, PcgState* st, RedBuf rb, bool f32)
+float* Multigrid::b32() const { return _buf->f[2].ptr; }
+
+const double* Multigrid::vcycle(const double* r, PcgState* st, RedBuf rb, bool f32,
+                                bool b32_ready)
 {
     Buffers& B = *_buf;
     const int last = (int)_lev.size() - 1;
@@ -318,7 +321,8 @@ const double* Multigrid::vcycle(const double* r, PcgState* st, RedBuf rb, bool f
             if (!B.f[i].ptr)
                 B.f[i].alloc(_g.total, _s);
         float *x = B.f[0].ptr, *t = B.f[1].ptr;
-        launch_mg_cast32(_g, r, B.f[2].ptr, st, _s);
+        if (!b32_ready)
+            launch_mg_cast32(_g, r, B.f[2].ptr, st, _s);
         if (_nu == 1)
             launch_mg0_first_f32(_g, _wt[0], B.f[2].ptr, x, st, _s);
         else

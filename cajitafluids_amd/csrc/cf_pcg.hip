@@ -280,7 +280,11 @@ struct DirEpiT
 // next plane after each step's barrier instead (kLate), into one buffer,
 // which keeps it at 4 waves per SIMD in FP32 (double-buffered it would take
 // 143 VGPRs, 3 waves).
-template <class T, int CPL, int XM>
+//
+// R32: the update also writes (float) r, the FP32 right-hand side of a
+// multigrid V-cycle whose finest level runs in FP32 (the V-cycle then needs
+// no cast of its own).
+template <class T, int CPL, int XM, bool R32 = false>
 struct UpdEpiT
 {
     const Geo& g;
@@ -289,6 +293,7 @@ struct UpdEpiT
     T alpha;
     const T* __restrict__ p_lag; // X_PAIR: the previous direction
     T alpha_lag;
+    float* __restrict__ r32;     // R32: (float) r
     double hi[2] = {0.0, 0.0}, lo[2] = {0.0, 0.0};
     static constexpr bool kReadX = XM != X_SKIP;
     static constexpr bool kLate = XM == X_PAIR;
@@ -380,6 +385,13 @@ struct UpdEpiT
             vstore_valid<T, CPL>(x + at, xo, c.valid[q]);
         }
         vstore_valid<T, CPL>(r + at, ro, c.valid[q]);
+        if constexpr (R32) {
+            float rf[CPL];
+#pragma unroll
+            for (int j = 0; j < CPL; ++j)
+                rf[j] = (float)ro[j];
+            vstore_valid<float, CPL>(r32 + at, rf, c.valid[q]);
+        }
 #pragma unroll
         for (int j = 0; j < CPL; ++j) {
             if (!c.valid[q][j])
@@ -409,16 +421,17 @@ __global__ __launch_bounds__(256) void k_pcg_dir_spmv3(
                    [&](double* sm) { st->alpha = st->rho / sm[0]; });
 }
 
-template <class T, int CPL, bool PX, int XM>
+template <class T, int CPL, bool PX, int XM, bool R32>
 __global__ __launch_bounds__(256) void k_pcg_update3(
     Geo g, Tile3Grid tg, T* __restrict__ x, T* __restrict__ r,
-    const T* __restrict__ p, const T* __restrict__ p_lag, PcgState* st, RedBuf rb)
+    const T* __restrict__ p, const T* __restrict__ p_lag, float* __restrict__ r32,
+    PcgState* st, RedBuf rb)
 {
     using namespace tile3;
     if (st->done)
         return;
     __shared__ __align__(16) T pl[2][TY + 2][prow<CPL>()];
-    UpdEpiT<T, CPL, XM> epi{g, x, r, (T)st->alpha, p_lag, (T)st->alpha_lag};
+    UpdEpiT<T, CPL, XM, R32> epi{g, x, r, (T)st->alpha, p_lag, (T)st->alpha_lag, r32};
     // downwards: the direction kernel marches up, so each kernel starts on
     // the planes the other touched last (Infinity Cache hits; FP64 256^3:
     // update 127 -> 121 us, direction 101 -> 95 us)
@@ -664,34 +677,32 @@ static void dir_spmv3_t(const Geo& g, const T* r, const T* p_old, T* p_new, PcgS
                        tile3_grid<cpl_of<T>()>(g), r, p_old, p_new, st, rb);
 }
 
-template <class T, bool PX, int XM>
-static void update3_t(const Geo& g, T* x, T* r, const T* p, const T* p_lag, PcgState* st,
-                      RedBuf rb, hipStream_t s)
+template <class T, bool PX, int XM, bool R32>
+static void update3_t(const Geo& g, T* x, T* r, const T* p, const T* p_lag, float* r32,
+                      PcgState* st, RedBuf rb, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_pcg_update3<T, cpl_of<T>(), PX, XM>),
+    hipLaunchKernelGGL((k_pcg_update3<T, cpl_of<T>(), PX, XM, R32>),
                        dim3(tile3_blocks<cpl_of<T>()>(g)), dim3(kBlock), 0, s, g,
-                       tile3_grid<cpl_of<T>()>(g), x, r, p, p_lag, st, rb);
+                       tile3_grid<cpl_of<T>()>(g), x, r, p, p_lag, r32, st, rb);
 }
 
-template <class T, bool PX>
-static void update3_x(const Geo& g, T* x, T* r, const T* p, const T* p_lag, PcgState* st,
-                      RedBuf rb, hipStream_t s, int xmode)
+template <class T, bool PX, bool R32>
+static void update3_x(const Geo& g, T* x, T* r, const T* p, const T* p_lag, float* r32,
+                      PcgState* st, RedBuf rb, hipStream_t s, int xmode)
 {
     if (xmode == X_EACH) {
-        update3_t<T, PX, X_EACH>(g, x, r, p, p_lag, st, rb, s);
+        update3_t<T, PX, X_EACH, R32>(g, x, r, p, p_lag, r32, st, rb, s);
         return;
     }
-    {
-        if (xmode == X_SKIP) {
-            update3_t<T, PX, X_SKIP>(g, x, r, p, p_lag, st, rb, s);
-            return;
-        }
-        if (xmode == X_PAIR) {
-            if (!p_lag || p_lag == p)
-                throw CfError(CF_EINVAL, "X_PAIR update needs the previous direction buffer");
-            update3_t<T, PX, X_PAIR>(g, x, r, p, p_lag, st, rb, s);
-            return;
-        }
+    if (xmode == X_SKIP) {
+        update3_t<T, PX, X_SKIP, R32>(g, x, r, p, p_lag, r32, st, rb, s);
+        return;
+    }
+    if (xmode == X_PAIR) {
+        if (!p_lag || p_lag == p)
+            throw CfError(CF_EINVAL, "X_PAIR update needs the previous direction buffer");
+        update3_t<T, PX, X_PAIR, R32>(g, x, r, p, p_lag, r32, st, rb, s);
+        return;
     }
     throw CfError(CF_EINVAL, "x mode not available for this PCG update");
 }
@@ -714,20 +725,29 @@ void launch_pcg_dir_spmv(const Geo& g, const T* r, const T* p_old, T* p_new,
 
 template <class T>
 void launch_pcg_update(const Geo& g, T* x, T* r, const T* p, PcgState* st,
-                       RedBuf rb, hipStream_t s, int xmode, const T* p_lag)
+                       RedBuf rb, hipStream_t s, int xmode, const T* p_lag, float* r32)
 {
     if (g.dim == 2) {
-        if (xmode != X_EACH)
-            throw CfError(CF_EINVAL, "the 2-D PCG update takes x every iteration");
+        if (xmode != X_EACH || r32)
+            throw CfError(CF_EINVAL, "the 2-D PCG update takes x every iteration and "
+                                     "writes no FP32 copy of r");
         CF_STREAM_LAUNCH((k_pcg_update<2, T>), g, x, r, p, st, rb);
         return;
     }
-    if constexpr (cpl_of<T>() > 1)
+    if constexpr (cpl_of<T>() > 1) {
+        if (r32)
+            throw CfError(CF_EINVAL, "the FP32 PCG update writes no FP32 copy of r");
         if (g.periodic[0]) {
-            update3_x<T, true>(g, x, r, p, p_lag, st, rb, s, xmode);
+            update3_x<T, true, false>(g, x, r, p, p_lag, nullptr, st, rb, s, xmode);
             return;
         }
-    update3_x<T, false>(g, x, r, p, p_lag, st, rb, s, xmode);
+        update3_x<T, false, false>(g, x, r, p, p_lag, nullptr, st, rb, s, xmode);
+    } else {
+        if (r32)
+            update3_x<T, false, true>(g, x, r, p, p_lag, r32, st, rb, s, xmode);
+        else
+            update3_x<T, false, false>(g, x, r, p, p_lag, nullptr, st, rb, s, xmode);
+    }
 }
 
 template <class T>
@@ -749,9 +769,10 @@ template void launch_pcg_dir_spmv<float>(const Geo&, const float*, const float*,
                                          float*, PcgState*, RedBuf, hipStream_t);
 template void launch_pcg_update<double>(const Geo&, double*, double*,
                                         const double*, PcgState*, RedBuf,
-                                        hipStream_t, int, const double*);
+                                        hipStream_t, int, const double*, float*);
 template void launch_pcg_update<float>(const Geo&, float*, float*, const float*,
-                                       PcgState*, RedBuf, hipStream_t, int, const float*);
+                                       PcgState*, RedBuf, hipStream_t, int, const float*,
+                                       float*);
 template void launch_pcg_xfix<double>(const Geo&, double*, const double*, const PcgState*,
                                       hipStream_t);
 template void launch_pcg_xfix<float>(const Geo&, float*, const float*, const PcgState*,

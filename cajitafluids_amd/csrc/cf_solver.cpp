@@ -551,12 +551,15 @@ cf_status VelocityCorrector::solveMg(const double* b, double* x, cf_solve_stats*
             launch_pcg_dir_spmv<double>(_g_unit, z, _p[cur].ptr, _p[cur ^ 1].ptr, _st, rb, _s);
             _timer.end(t0, _s, TK_DIR_SPMV, it);
             int t1 = _timer.begin_iter(_s, it);
+            // with the FP32 V-cycle the update also writes its right-hand
+            // side, (float) r
             launch_pcg_update<double>(_g, x, _r.ptr, _p[cur ^ 1].ptr, _st, rb, _s,
-                                      xmode(pairs, launched + c), _p[cur].ptr);
+                                      xmode(pairs, launched + c), _p[cur].ptr,
+                                      f32 ? _mg->b32() : nullptr);
             _timer.end(t1, _s, TK_UPDATE, it);
             // the V-cycle prepares iteration it + 1 (a no-op after the stop)
             int t2 = _timer.begin_iter(_s, it + 1);
-            z = _mg->vcycle(_r.ptr, _st, rb, f32);
+            z = _mg->vcycle(_r.ptr, _st, rb, f32, f32);
             _timer.end(t2, _s, TK_VCYCLE, it + 1);
             cur ^= 1;
         }
