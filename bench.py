@@ -603,6 +603,16 @@ def adjoint_legs():
                                           lambda s: s.get(0))
     out["periodic_box_384_mg"] = adjoint_leg(with_mg(c5), K.PERIODIC_BOX_STEPS, K.PERIODIC_BOX_CKPT,
                                              init5, lambda s: s.get(0))
+    # the workload's FP32-refined precision with the multigrid solver: the
+    # V-cycle's finest level in FP32
+    c5m = with_mg(c5)
+    c5m["precision"] = K.FP32_REFINED
+    out["periodic_box_384_mg_fp32"] = adjoint_leg(c5m, K.PERIODIC_BOX_STEPS,
+                                                  K.PERIODIC_BOX_CKPT, init5, lambda s: s.get(0))
+    out["periodic_box_384_mg_fp32"]["grad_u_rel_diff_vs_mg_fp64"] = float(
+        np.linalg.norm(out["periodic_box_384_mg_fp32"]["_du"] - out["periodic_box_384_mg"]["_du"])
+        / np.linalg.norm(out["periodic_box_384_mg"]["_du"]))
+    out["periodic_box_384_mg_fp32"].pop("_du", None)
     compare_mg(out, "periodic_box_384")
     return out
 

@@ -178,12 +178,17 @@ def _adjoint_case(name):
     return cfg, 3, 2
 
 
-@pytest.mark.parametrize("name", ["2d_tank", "3d_periodic_box_12"])
+@pytest.mark.parametrize("name", ["2d_tank", "3d_periodic_box_12", "3d_periodic_box_12_fp32"])
 def test_mg_adjoint_matches_oracle(cf, oracle_mod, name):
     """The reverse mode's pressure solves go through the same preconditioner
-    on both sides."""
-    cfg, nsteps, ckpt = _adjoint_case(name)
+    on both sides.  With the FP32 finest level (fp32) the library's forward
+    and reverse solves use it, the oracle's reverse solves the FP64 V-cycle;
+    both CGs reach 1e-12, so the gradients agree to 1e-9."""
+    fp32 = name.endswith("_fp32")
+    cfg, nsteps, ckpt = _adjoint_case(name[:-5] if fp32 else name)
     cfg["preconditioner"] = K.PRECOND_MG
+    if fp32:
+        cfg["precision"] = K.FP32_REFINED
     dim = cfg["dim"]
     rng = np.random.default_rng(7)
     init = {}
@@ -206,7 +211,7 @@ def test_mg_adjoint_matches_oracle(cf, oracle_mod, name):
     for e, k in enumerate(("dq0", "du", "dv", "dw")[:dim + 1]):
         _, lo, hi = cf.layout(cfg, e)
         sl = (slice(lo[2], hi[2]), slice(lo[1], hi[1]), slice(lo[0], hi[0]))
-        assert rel_l2(g[k][sl], go[k][sl]) <= 1e-10, k
+        assert rel_l2(g[k][sl], go[k][sl]) <= (1e-9 if fp32 else 1e-10), k
     orc.close()
 
 
