@@ -294,7 +294,7 @@ def test_dye_tank_adjoint_512(cf):
                        atol=GRAD_TOL * float(np.abs(dref).max())), (g["dvel"], dref)
 
 
-def test_dye_tank_fd_32_coordinates(cf):
+def test_dye_tank_fd_32_coordinates(cf, capsys):
     """The 512^2 dye-tank gradient w.r.t. 32 seeded initial-velocity
     coordinates (seed 44) against central differences, eps 1e-6, with the
     PCG run to 1e-14 in every solve so that its stopping rule adds no jump.
@@ -353,7 +353,11 @@ def test_dye_tank_fd_32_coordinates(cf):
         fd = (lp - lm) / (2 * eps)
         adj = float(g["du" if e == 1 else "dv"][gi])
         floor = 4.0 * max(abs(lp), abs(lm)) * 2.0 ** -52 / eps
-        print(f"coord {n:2d} entity {e} {gi}: adjoint {adj:+.10e} fd {fd:+.10e}", flush=True)
+        # a progress line per coordinate on the real output (about 10 s
+        # apart): the whole test takes minutes
+        with capsys.disabled():
+            print(f"fd-32 coord {n:2d} entity {e} {gi}: adjoint {adj:+.10e} "
+                  f"fd {fd:+.10e}", flush=True)
         worst = max(worst, abs(adj - fd) / (GRAD_TOL * abs(fd) + floor))
     assert worst <= 1.0, worst
 
