@@ -87,6 +87,10 @@ void launch_pcg_init32(const Geo& g, const float* r, float* d, PcgState* st,
 template <class T>
 void launch_pcg_dir_spmv(const Geo& g, const T* r, const T* p_old, T* p_new,
                          PcgState* st, RedBuf rb, hipStream_t s);
+// The FP64 3-D direction kernel on a unit-Jacobi geometry reading an FP32 z
+// (an FP32 multigrid V-cycle's output): p_new = fma(beta, p_old, (double) z).
+void launch_pcg_dir_spmv_z32(const Geo& g, const float* z, const double* p_old, double* p_new,
+                             PcgState* st, RedBuf rb, hipStream_t s);
 // The update's treatment of x.  The 3-D tile kernels can leave x alone on
 // every second iteration and apply both steps on the next, in the order of
 // two single steps (same bits):

@@ -159,8 +159,8 @@ __global__ __launch_bounds__(256) void k_mg0_smooth(Geo g, MgTab wd, const doubl
 // from gw's Jacobi table; RZ also accumulates r.z = b . xo.  RAW: b is the
 // marched operand (the first sweep), taken from the tile's loads.  T is the
 // level's vector type.  On an FP32 level the RZ sweep (level 0's last) reads
-// the FP64 right-hand side r and writes z in FP64 (TB = double): the sweep
-// computes in FP32 from (float) r, and r.z uses r itself.
+// the FP64 right-hand side r (TB = double): the sweep computes in FP32 from
+// (float) r, and r.z uses r itself and (double) z.
 template <class T, bool RZ>
 using MgRhsT = std::conditional_t<std::is_same<T, float>::value && RZ, double, T>;
 
@@ -171,7 +171,7 @@ struct MgSweepEpi
     static constexpr bool kRaw = RAW;
     const Geo& gw;
     const TB* __restrict__ b;
-    TB* __restrict__ xo;
+    T* __restrict__ xo;
     double hi[1] = {0.0}, lo[1] = {0.0};
     TB bb[2][tile3::RY]; // b of a plane, alternating with the march's steps
     template <int B, class PL>
@@ -199,7 +199,7 @@ struct MgSweepEpi
     {
         const T w = dinv_pick<T>(gw, c.wy[q] + m.wz, c.wx[0]);
         const T out = fma_t<T>(w, (T)bb[B][q] - y[0], pc[0]);
-        xo[c.colb[q] + m.off] = (TB)out;
+        xo[c.colb[q] + m.off] = out;
         if constexpr (RZ)
             dd_acc_prod<false>(hi[0], lo[0], (double)bb[B][q], (double)out);
     }
@@ -211,9 +211,8 @@ template <class T, int MODE, bool RZ>
 __global__ __launch_bounds__(256) void k_mg_smooth3(Geo g, Geo gw, Tile3Grid tg,
                                                     const T* __restrict__ xi,
                                                     const MgRhsT<T, RZ>* __restrict__ b,
-                                                    Tile3Parent par,
-                                                    MgRhsT<T, RZ>* __restrict__ xo, PcgState* st,
-                                                    RedBuf rb)
+                                                    Tile3Parent par, T* __restrict__ xo,
+                                                    PcgState* st, RedBuf rb)
 {
     using namespace tile3;
     if (st->done)
@@ -860,8 +859,8 @@ void launch_mg_smooth3(const Geo& g, const Geo& gw, int src, bool rz, const doub
 
 void launch_mg_smooth3_f32(const Geo& g, const Geo& gw, int src, bool rz, const float* xi,
                            const float* b32, const double* r64, const double* xc,
-                           MgParentMap pm, double alpha, float* xo, double* z64, PcgState* st,
-                           RedBuf rb, hipStream_t s)
+                           MgParentMap pm, double alpha, float* xo, PcgState* st, RedBuf rb,
+                           hipStream_t s)
 {
     const Tile3Grid tg = mg_tile_grid(g);
     const dim3 grid((tg.work + 7) / 8 * 8), blk(kBlock);
@@ -875,12 +874,12 @@ void launch_mg_smooth3_f32(const Geo& g, const Geo& gw, int src, bool rz, const 
         CF_MG_SMOOTH3F(1, false, b32, xo);
     } else if (src == MG_PLAIN) {
         if (rz)
-            CF_MG_SMOOTH3F(0, true, r64, z64);
+            CF_MG_SMOOTH3F(0, true, r64, xo);
         else
             CF_MG_SMOOTH3F(0, false, b32, xo);
     } else {
         if (rz)
-            CF_MG_SMOOTH3F(2, true, r64, z64);
+            CF_MG_SMOOTH3F(2, true, r64, xo);
         else
             CF_MG_SMOOTH3F(2, false, b32, xo);
     }

@@ -113,18 +113,18 @@ void launch_mg_resrestrict3(const Geo& g, const double* x, const double* b, doub
                             MgParentMap pm, const PcgState* st, hipStream_t s);
 
 // FP32 level 0 (3-D, marched): the same sweeps and restriction on FP32
-// vectors.  b32 = (float) r (launch_mg_cast32); a sweep with rz reads r64
-// instead of b32, writes z64 = (double) x and reduces r.z; the restriction
-// sums the FP32 residuals as doubles into the FP64 level 1; the
-// prolongation rounds the FP64 parent to FP32.
+// vectors.  b32 = (float) r (launch_mg_cast32, or the PCG update); a sweep
+// with rz reads r64 instead of b32 (its sweep uses (float) r64) and reduces
+// r.z with the FP64 r; the restriction sums the FP32 residuals as doubles
+// into the FP64 level 1; the prolongation rounds the FP64 parent to FP32.
 void launch_mg_cast32(const Geo& g, const double* r, float* b32, const PcgState* st,
                       hipStream_t s);
 void launch_mg0_first_f32(const Geo& g, const MgTab& wd0, const float* b, float* x,
                           const PcgState* st, hipStream_t s);
 void launch_mg_smooth3_f32(const Geo& g, const Geo& gw, int src, bool rz, const float* xi,
                            const float* b32, const double* r64, const double* xc,
-                           MgParentMap pm, double alpha, float* xo, double* z64, PcgState* st,
-                           RedBuf rb, hipStream_t s);
+                           MgParentMap pm, double alpha, float* xo, PcgState* st, RedBuf rb,
+                           hipStream_t s);
 void launch_mg_resrestrict3_f32(const Geo& g, const float* x, const float* b, double* bc,
                                 MgParentMap pm, const PcgState* st, hipStream_t s);
 
@@ -153,13 +153,13 @@ class Multigrid
     ~Multigrid();
     // z = B r (one V-cycle), then rho / beta from r.z; returns the level-0
     // buffer that holds z (one of the two the hierarchy owns).  Kernels
-    // return at once when st->done.  f32: level 0 on FP32 vectors (3-D
-    // grids with more than 2 cells per axis, f32_supported()), the coarser
-    // levels and z in FP64.
-    // b32_ready: the caller has written (float) r into b32() (the PCG
-    // update does), so the FP32 V-cycle skips its own cast.
-    const double* vcycle(const double* r, PcgState* st, RedBuf rb, bool f32 = false,
-                         bool b32_ready = false);
+    // return at once when st->done.
+    const double* vcycle(const double* r, PcgState* st, RedBuf rb);
+    // The same with level 0 on FP32 vectors (3-D grids with more than 2
+    // cells per axis, f32_supported()), the coarser levels in FP64; z comes
+    // back in FP32.  b32_ready: the caller has written (float) r into b32()
+    // (the PCG update does), so the V-cycle skips its own cast.
+    const float* vcycle32(const double* r, PcgState* st, RedBuf rb, bool b32_ready);
     bool f32_supported() const { return _nt >= 1 && _lev.size() >= 2; }
     // the FP32 level 0's right-hand side (null before the first FP32 V-cycle)
     float* b32() const;
@@ -168,6 +168,7 @@ class Multigrid
 
   private:
     struct Buffers;
+    const void* cycle(const double* r, PcgState* st, RedBuf rb, bool f32, bool b32_ready);
     void first(int l, const double* b, double* x, PcgState* st);
     void smooth(int l, int src, bool rz, const double* xi, const double* b, double* xo,
                 PcgState* st, RedBuf rb);

@@ -281,13 +281,24 @@ void Multigrid::smooth32(int src, bool rz, const float* xi, const double* r, flo
 {
     Buffers& B = *_buf;
     launch_mg_smooth3_f32(_lg[0], _lgw[0], src, rz, xi, B.f[2].ptr, r, B.res[1], _pm[0],
-                          _lev[0].alpha, xo, B.z[0].ptr, st, rb, _s);
+                          _lev[0].alpha, xo, st, rb, _s);
 }
 
 float* Multigrid::b32() const { return _buf->f[2].ptr; }
 
-const double* Multigrid::vcycle(const double* r, PcgState* st, RedBuf rb, bool f32,
-                                bool b32_ready)
+const double* Multigrid::vcycle(const double* r, PcgState* st, RedBuf rb)
+{
+    return static_cast<const double*>(cycle(r, st, rb, false, false));
+}
+
+const float* Multigrid::vcycle32(const double* r, PcgState* st, RedBuf rb, bool b32_ready)
+{
+    return static_cast<const float*>(cycle(r, st, rb, true, b32_ready));
+}
+
+// One V-cycle; returns z (double, or float with f32).
+const void* Multigrid::cycle(const double* r, PcgState* st, RedBuf rb, bool f32,
+                             bool b32_ready)
 {
     Buffers& B = *_buf;
     const int last = (int)_lev.size() - 1;
@@ -366,8 +377,8 @@ const double* Multigrid::vcycle(const double* r, PcgState* st, RedBuf rb, bool f
         }
     }
     if (f32) {
-        // level 0 in FP32: the last sweep writes z = (double) x to z[0] and
-        // reduces r.z from the FP64 r
+        // level 0 in FP32: the last sweep writes z and reduces r.z from the
+        // FP64 r; z stays in FP32 (the direction kernel widens it)
         float* xf = x32;
         float* tf = xf == B.f[0].ptr ? B.f[1].ptr : B.f[0].ptr;
         smooth32(MG_PROLONG, _nu == 1, xf, r, tf, st, rb);
@@ -376,7 +387,7 @@ const double* Multigrid::vcycle(const double* r, PcgState* st, RedBuf rb, bool f
             smooth32(MG_PLAIN, it == _nu - 1, xf, r, tf, st, rb);
             std::swap(xf, tf);
         }
-        return B.z[0].ptr;
+        return xf;
     }
     return x;
 }

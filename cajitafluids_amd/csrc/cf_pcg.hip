@@ -403,9 +403,11 @@ struct UpdEpiT
     }
 };
 
-template <class T, int CPL, bool PX>
+// TR: the element type of r (float: an FP32 multigrid V-cycle's z, widened
+// on load; its values are the FP64 z's exactly).
+template <class T, int CPL, bool PX, class TR = T>
 __global__ __launch_bounds__(256) void k_pcg_dir_spmv3(
-    Geo g, Tile3Grid tg, const T* __restrict__ r, const T* __restrict__ p_old,
+    Geo g, Tile3Grid tg, const TR* __restrict__ r, const T* __restrict__ p_old,
     T* __restrict__ p_new, PcgState* st, RedBuf rb)
 {
     using namespace tile3;
@@ -415,7 +417,7 @@ __global__ __launch_bounds__(256) void k_pcg_dir_spmv3(
     const bool first = st->iter == 0;
     const T beta = (T)st->beta;
     DirEpiT<T, CPL> epi{g, p_new};
-    tile3_march<T, true, CPL, PX>(g, tg, r, p_old, first, beta, pl, epi);
+    tile3_march<T, true, CPL, PX, false, TR>(g, tg, r, p_old, first, beta, pl, epi);
     dd_norm(epi.hi[0], epi.lo[0]);
     grid_reduce<1>(epi.hi, epi.lo, rb,
                    [&](double* sm) { st->alpha = st->rho / sm[0]; });
@@ -705,6 +707,15 @@ static void update3_x(const Geo& g, T* x, T* r, const T* p, const T* p_lag, floa
         return;
     }
     throw CfError(CF_EINVAL, "x mode not available for this PCG update");
+}
+
+void launch_pcg_dir_spmv_z32(const Geo& g, const float* z, const double* p_old, double* p_new,
+                             PcgState* st, RedBuf rb, hipStream_t s)
+{
+    if (g.dim != 3)
+        throw CfError(CF_EINVAL, "an FP32 z is a 3-D multigrid V-cycle's");
+    hipLaunchKernelGGL((k_pcg_dir_spmv3<double, 1, false, float>), dim3(tile3_blocks<1>(g)),
+                       dim3(kBlock), 0, s, g, tile3_grid<1>(g), z, p_old, p_new, st, rb);
 }
 
 template <class T>

@@ -537,7 +537,13 @@ cf_status VelocityCorrector::solveMg(const double* b, double* x, cf_solve_stats*
                                    "than 2 cells along every axis");
     RedBuf rb{_red_partials.ptr, _red_counter.ptr};
     launch_mgpcg_init(_g, b, x, _r.ptr, _st, _cfg.cg_tol, _cfg.cg_max_iter, rb, _s);
-    const double* z = _mg->vcycle(_r.ptr, _st, rb, f32);
+    // the V-cycle's z: FP64, or FP32 from the FP32 finest level
+    const double* z = nullptr;
+    const float* z32 = nullptr;
+    if (f32)
+        z32 = _mg->vcycle32(_r.ptr, _st, rb, false);
+    else
+        z = _mg->vcycle(_r.ptr, _st, rb);
     PcgState hs;
     readState(hs);
     int cur = 0, launched = 0;
@@ -548,7 +554,12 @@ cf_status VelocityCorrector::solveMg(const double* b, double* x, cf_solve_stats*
         for (int c = 0; c < batch; ++c) {
             const int it = it0 + launched + c + 1;
             int t0 = _timer.begin_iter(_s, it);
-            launch_pcg_dir_spmv<double>(_g_unit, z, _p[cur].ptr, _p[cur ^ 1].ptr, _st, rb, _s);
+            if (f32)
+                launch_pcg_dir_spmv_z32(_g_unit, z32, _p[cur].ptr, _p[cur ^ 1].ptr, _st, rb,
+                                        _s);
+            else
+                launch_pcg_dir_spmv<double>(_g_unit, z, _p[cur].ptr, _p[cur ^ 1].ptr, _st, rb,
+                                            _s);
             _timer.end(t0, _s, TK_DIR_SPMV, it);
             int t1 = _timer.begin_iter(_s, it);
             // with the FP32 V-cycle the update also writes its right-hand
@@ -559,7 +570,10 @@ cf_status VelocityCorrector::solveMg(const double* b, double* x, cf_solve_stats*
             _timer.end(t1, _s, TK_UPDATE, it);
             // the V-cycle prepares iteration it + 1 (a no-op after the stop)
             int t2 = _timer.begin_iter(_s, it + 1);
-            z = _mg->vcycle(_r.ptr, _st, rb, f32, f32);
+            if (f32)
+                z32 = _mg->vcycle32(_r.ptr, _st, rb, true);
+            else
+                z = _mg->vcycle(_r.ptr, _st, rb);
             _timer.end(t2, _s, TK_VCYCLE, it + 1);
             cur ^= 1;
         }

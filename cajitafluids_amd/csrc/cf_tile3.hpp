@@ -403,12 +403,15 @@ struct Plane3
     T b[tile3::NS][CPL];
 };
 
-template <class T, int MODE, int CPL, bool PX = false>
+// TA: the element type of va (an FP32 operand widened to T on load, one
+// cell per lane only).
+template <class T, int MODE, int CPL, bool PX = false, class TA = T>
 __device__ __forceinline__ void tile3_load(const Tile3Ctx<CPL>& c, const Plane3Meta& m,
-                                           const T* __restrict__ va,
+                                           const TA* __restrict__ va,
                                            const T* __restrict__ vb, bool need_b,
                                            const Tile3Parent& par, Plane3<T, MODE, CPL>& P)
 {
+    static_assert(std::is_same<TA, T>::value || CPL == 1, "a widened operand: one cell per lane");
     constexpr bool DIR = MODE == 1;
     static_assert(MODE != 2 || CPL == 1, "the prolongation mode runs one cell per lane");
 #pragma unroll
@@ -422,14 +425,14 @@ __device__ __forceinline__ void tile3_load(const Tile3Ctx<CPL>& c, const Plane3M
         if constexpr (MODE == 2) {
             const int64_t pat = par.base + (c.sgi[s] >> 1) + par.sy * (c.sgj[s] >> 1) +
                                 par.sz * (m.zc >> 1);
-            P.a[s][0] = load_any<T>(va, ok, at, c.safe);
+            P.a[s][0] = (T)load_any<TA>(va, ok, at, c.safe);
             // the parent level is FP64; an FP32 level rounds its value
             P.b[s][0] = (T)load_any<double>(par.xc, ok, pat, par.base);
             continue;
         } else if constexpr (CPL == 1) {
             // the direction's old value is read on the first iteration too
             // (and ignored there): a load on every path
-            P.a[s][0] = load_any<T>(va, ok, at, c.safe);
+            P.a[s][0] = (T)load_any<TA>(va, ok, at, c.safe);
             if constexpr (DIR)
                 P.b[s][0] = load_any<T>(vb, ok, at, c.safe);
             continue;
@@ -613,9 +616,10 @@ __device__ __forceinline__ T tile3_apply(const Geo& g, const Tile3Ctx<CPL>& c,
 // marching the other way then starts on the planes its predecessor touched
 // last, which are still in the Infinity Cache.  The stencil sees the same
 // operands in the same order either way.
-template <class T, int DIR, int CPL, bool PX = false, bool REV = false, class Epi>
+template <class T, int DIR, int CPL, bool PX = false, bool REV = false, class TA = T,
+          class Epi>
 __device__ __forceinline__ void tile3_march(
-    const Geo& g, const Tile3Grid& tg, const T* __restrict__ va,
+    const Geo& g, const Tile3Grid& tg, const TA* __restrict__ va,
     const T* __restrict__ vb, bool first, T beta,
     T (*pl)[tile3::TY + 2][tile3::prow<CPL>()], Epi&& epi,
     const Tile3Parent par = Tile3Parent{nullptr, 0, 0, 0, 0.0})
@@ -627,9 +631,11 @@ __device__ __forceinline__ void tile3_march(
         return;
     const bool need_b = DIR == 2 || !first;
     // MODE 1 reads vb on every path (first iteration included, where its
-    // values are ignored); a caller without an old direction gets va's
-    if (vb == nullptr)
-        vb = va;
+    // values are ignored); a caller without an old direction gets va's (a
+    // caller with a widened va passes its vb)
+    if constexpr (std::is_same<TA, T>::value)
+        if (vb == nullptr)
+            vb = va;
     Plane3<T, DIR, CPL> P0, P1;
     // pm: the plane behind the march, pc: the current one, pp: the one ahead
     T pm[RY][CPL], pc[RY][CPL], pp[RY][CPL];
@@ -640,17 +646,17 @@ __device__ __forceinline__ void tile3_march(
     // prologue: the plane behind kbeg (own column only), kbeg into LDS, the
     // plane ahead in flight
     Plane3Meta mm = plane3_meta(g, kbeg - D);
-    tile3_load<T, DIR, CPL, PX>(c, mm, va, vb, need_b, par, P0);
+    tile3_load<T, DIR, CPL, PX, TA>(c, mm, va, vb, need_b, par, P0);
 #pragma unroll
     for (int q = 0; q < RY; ++q)
 #pragma unroll
         for (int j = 0; j < CPL; ++j)
             pm[q][j] = tile3_value<T, DIR, CPL>(g, c, mm, P0, q, j, first, beta);
     Plane3Meta mc = plane3_meta(g, kbeg);
-    tile3_load<T, DIR, CPL, PX>(c, mc, va, vb, need_b, par, P1);
+    tile3_load<T, DIR, CPL, PX, TA>(c, mc, va, vb, need_b, par, P1);
     tile3_finish<T, DIR, CPL>(g, c, mc, P1, first, beta, pl[0], pc);
     Plane3Meta mn = plane3_meta(g, kbeg + D);
-    tile3_load<T, DIR, CPL, PX>(c, mn, va, vb, need_b, par, P0);
+    tile3_load<T, DIR, CPL, PX, TA>(c, mn, va, vb, need_b, par, P0);
     if constexpr (epi_takes_raw<E>::value)
         epi.template take_raw<0>(P1);
     epi.template prefetch<0>(c, mc, kbeg);
@@ -665,7 +671,7 @@ __device__ __forceinline__ void tile3_march(
         // the last step loads the plane past the chunk again (an L2 hit,
         // never used) rather than branching around the loads
         const Plane3Meta mnn = plane3_meta(g, ahead2(k));
-        tile3_load<T, DIR, CPL, PX>(c, mnn, va, vb, need_b, par, Pl);
+        tile3_load<T, DIR, CPL, PX, TA>(c, mnn, va, vb, need_b, par, Pl);
         tile3_finish<T, DIR, CPL>(g, c, mn, Pf, first, beta, pl[S ^ 1], pp);
         if constexpr (epi_takes_raw<E>::value)
             epi.template take_raw<S ^ 1>(Pf);
@@ -708,7 +714,7 @@ __device__ __forceinline__ void tile3_march(
         int cur = 0;
         for (int s = 0, k = kbeg; s < c.k1 - c.k0; ++s, k += D) {
             const Plane3Meta mnn = plane3_meta(g, ahead2(k));
-            tile3_load<T, DIR, CPL, PX>(c, mnn, va, vb, need_b, par, P1);
+            tile3_load<T, DIR, CPL, PX, TA>(c, mnn, va, vb, need_b, par, P1);
             tile3_finish<T, DIR, CPL>(g, c, mn, P0, first, beta, pl[cur ^ 1], pp);
             epi.template prefetch<1>(c, mn, k + D);
 #pragma unroll
