@@ -164,7 +164,7 @@ __global__ __launch_bounds__(256) void k_mg0_smooth(Geo g, MgTab wd, const doubl
 template <class T, bool RZ>
 using MgRhsT = std::conditional_t<std::is_same<T, float>::value && RZ, double, T>;
 
-template <class T, bool RZ, bool RAW>
+template <class T, int CPL, bool RZ, bool RAW>
 struct MgSweepEpi
 {
     using TB = MgRhsT<T, RZ>;
@@ -173,41 +173,70 @@ struct MgSweepEpi
     const TB* __restrict__ b;
     T* __restrict__ xo;
     double hi[1] = {0.0}, lo[1] = {0.0};
-    TB bb[2][tile3::RY]; // b of a plane, alternating with the march's steps
+    TB bb[2][tile3::RY][CPL]; // b of a plane, alternating with the march's steps
     template <int B, class PL>
     __device__ void take_raw(const PL& P)
     {
 #pragma unroll
         for (int q = 0; q < tile3::RY; ++q)
-            bb[B][q] = (TB)P.a[q][0];
+#pragma unroll
+            for (int j = 0; j < CPL; ++j)
+                bb[B][q][j] = (TB)P.a[q][j];
     }
     template <int B>
-    __device__ void prefetch(const Tile3Ctx<1>& c, const Plane3Meta& m, int k)
+    __device__ void prefetch(const Tile3Ctx<CPL>& c, const Plane3Meta& m, int k)
     {
         if constexpr (!RAW) {
 #pragma unroll
             for (int q = 0; q < tile3::RY; ++q) {
                 const bool ok = k < c.k1 && c.valid[q][0];
-                bb[B][q] = load_any<TB>(b, ok, c.colb[q] + m.off, c.safe); // raw: read on valid rows only
+                if constexpr (CPL == 1) {
+                    bb[B][q][0] = load_any<TB>(b, ok, c.colb[q] + m.off, c.safe); // raw: read on valid rows only
+                } else {
+                    // an aligned pair on every path (raw, as above)
+                    const int64_t ap = ok ? c.colb[q] + m.off : c.safe;
+                    vload<TB, CPL>(b + ap, bb[B][q]);
+                }
             }
         }
     }
+    __device__ void shift()
+    {
+#pragma unroll
+        for (int q = 0; q < tile3::RY; ++q)
+#pragma unroll
+            for (int j = 0; j < CPL; ++j)
+                bb[0][q][j] = bb[1][q][j];
+    }
     __device__ void rotate() {}
     template <int B>
-    __device__ void cells(const Tile3Ctx<1>& c, const Plane3Meta& m, int q, const T (&pc)[1],
-                          const T (&y)[1])
+    __device__ void cells(const Tile3Ctx<CPL>& c, const Plane3Meta& m, int q, const T (&pc)[CPL],
+                          const T (&y)[CPL])
     {
-        const T w = dinv_pick<T>(gw, c.wy[q] + m.wz, c.wx[0]);
-        const T out = fma_t<T>(w, (T)bb[B][q] - y[0], pc[0]);
-        xo[c.colb[q] + m.off] = out;
-        if constexpr (RZ)
-            dd_acc_prod<false>(hi[0], lo[0], (double)bb[B][q], (double)out);
+        T out[CPL];
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            const T w = dinv_pick<T>(gw, c.wy[q] + m.wz, c.wx[j]);
+            out[j] = fma_t<T>(w, (T)bb[B][q][j] - y[j], pc[j]);
+        }
+        if constexpr (CPL == 1)
+            xo[c.colb[q] + m.off] = out[0];
+        else
+            vstore_valid<T, CPL>(xo + c.colb[q] + m.off, out, c.valid[q]);
+        if constexpr (RZ) {
+#pragma unroll
+            for (int j = 0; j < CPL; ++j)
+                if (c.valid[q][j])
+                    dd_acc_prod<false>(hi[0], lo[0], (double)bb[B][q][j], (double)out[j]);
+        }
     }
 };
 
 // MODE: 1 = first sweep from zero (values w b, read from b through gw's
-// Jacobi table), 0 = plain, 2 = after the coarse correction.
-template <class T, int MODE, bool RZ>
+// Jacobi table), 0 = plain, 2 = after the coarse correction.  CPL, PX: the
+// tile shape (two cells per lane for an FP32 level, with its periodic-x
+// variant).
+template <class T, int MODE, bool RZ, int CPL = 1, bool PX = false>
 __global__ __launch_bounds__(256) void k_mg_smooth3(Geo g, Geo gw, Tile3Grid tg,
                                                     const T* __restrict__ xi,
                                                     const MgRhsT<T, RZ>* __restrict__ b,
@@ -217,13 +246,13 @@ __global__ __launch_bounds__(256) void k_mg_smooth3(Geo g, Geo gw, Tile3Grid tg,
     using namespace tile3;
     if (st->done)
         return;
-    __shared__ T pl[2][TY + 2][prow<1>()];
-    MgSweepEpi<T, RZ, MODE == 1> epi{gw, b, xo};
+    __shared__ __align__(16) T pl[2][TY + 2][prow<CPL>()];
+    MgSweepEpi<T, CPL, RZ, MODE == 1> epi{gw, b, xo};
     if constexpr (MODE == 1) {
         static_assert(std::is_same<MgRhsT<T, RZ>, T>::value, "the first sweep marches b itself");
-        tile3_march<T, 1, 1>(gw, tg, b, nullptr, true, (T)0, pl, epi);
+        tile3_march<T, 1, CPL, PX>(gw, tg, b, nullptr, true, (T)0, pl, epi);
     } else {
-        tile3_march<T, MODE, 1>(g, tg, xi, nullptr, false, (T)par.alpha, pl, epi, par);
+        tile3_march<T, MODE, CPL, PX>(g, tg, xi, nullptr, false, (T)par.alpha, pl, epi, par);
     }
     if constexpr (RZ) {
         dd_norm(epi.hi[0], epi.lo[0]);
@@ -240,48 +269,77 @@ __global__ __launch_bounds__(256) void k_mg_smooth3(Geo g, Geo gw, Tile3Grid tg,
 // (children in x, y order), and after the second plane of a z pair (or the
 // last plane) writes the parent.  rotate() runs on every lane of an active
 // block, so the lane exchange is not under divergent control.
-template <class T>
+template <class T, int CPL = 1>
 struct MgRestrictEpi
 {
     const Geo& g;
     const T* __restrict__ b;
     double* __restrict__ bc;
     MgParentMap pm;
-    const Tile3Ctx<1>* ctx = nullptr;
+    const Tile3Ctx<CPL>* ctx = nullptr;
     // b of a plane, alternating with the march's steps; the residuals of
     // an FP32 level are summed as doubles
-    T bb[2][tile3::RY], r[tile3::RY] = {};
+    T bb[2][tile3::RY][CPL], r[tile3::RY][CPL] = {};
     double sum[tile3::RY / 2] = {};
     int knext = -1, kcur = -1;
     template <int B>
-    __device__ void prefetch(const Tile3Ctx<1>& c, const Plane3Meta& m, int k)
+    __device__ void prefetch(const Tile3Ctx<CPL>& c, const Plane3Meta& m, int k)
     {
         ctx = &c;
         knext = k;
 #pragma unroll
         for (int q = 0; q < tile3::RY; ++q) {
             const bool ok = k < c.k1 && c.valid[q][0];
-            bb[B][q] = load_any<T>(b, ok, c.colb[q] + m.off, c.safe); // raw: read on valid rows only
+            if constexpr (CPL == 1) {
+                bb[B][q][0] = load_any<T>(b, ok, c.colb[q] + m.off, c.safe); // raw: read on valid rows only
+            } else {
+                const int64_t ap = ok ? c.colb[q] + m.off : c.safe;
+                vload<T, CPL>(b + ap, bb[B][q]);
+            }
         }
     }
-    template <int B>
-    __device__ void cells(const Tile3Ctx<1>&, const Plane3Meta&, int q, const T (&)[1],
-                          const T (&y)[1])
+    __device__ void shift()
     {
-        r[q] = bb[B][q] - y[0];
+#pragma unroll
+        for (int q = 0; q < tile3::RY; ++q)
+#pragma unroll
+            for (int j = 0; j < CPL; ++j)
+                bb[0][q][j] = bb[1][q][j];
     }
+    template <int B>
+    __device__ void cells(const Tile3Ctx<CPL>&, const Plane3Meta&, int q, const T (&)[CPL],
+                          const T (&y)[CPL])
+    {
+#pragma unroll
+        for (int j = 0; j < CPL; ++j)
+            r[q][j] = bb[B][q][j] - y[j];
+    }
+    // the children of a parent in z, y, x order: the x pair is an even lane
+    // and the next one (CPL = 1) or one lane's two cells (CPL = 2)
     __device__ void rotate()
     {
-        const Tile3Ctx<1>& c = *ctx;
+        const Tile3Ctx<CPL>& c = *ctx;
         if (kcur >= 0) {
-            const int gi = c.i0 + c.lane;
-            const bool even = (c.lane & 1) == 0;
-            const bool x1 = gi + 1 < g.n[0];
+            const int gi = c.i0 + c.lane * CPL;
+            const bool lead = CPL == 2 || (c.lane & 1) == 0;
 #pragma unroll
             for (int p = 0; p < tile3::RY / 2; ++p) {
-                const double a0 = (double)r[2 * p], a1 = (double)r[2 * p + 1];
-                const double b0 = __shfl_down(a0, 1), b1 = __shfl_down(a1, 1);
-                if (even && c.valid[2 * p][0]) {
+                double a0, a1, b0, b1;
+                bool x1;
+                if constexpr (CPL == 1) {
+                    a0 = (double)r[2 * p][0];
+                    a1 = (double)r[2 * p + 1][0];
+                    b0 = __shfl_down(a0, 1);
+                    b1 = __shfl_down(a1, 1);
+                    x1 = gi + 1 < g.n[0];
+                } else {
+                    a0 = (double)r[2 * p][0];
+                    b0 = (double)r[2 * p][1];
+                    a1 = (double)r[2 * p + 1][0];
+                    b1 = (double)r[2 * p + 1][1];
+                    x1 = gi + 1 < g.n[0];
+                }
+                if (lead && c.valid[2 * p][0]) {
                     sum[p] = sum[p] + a0;
                     if (x1)
                         sum[p] = sum[p] + b0;
@@ -295,7 +353,7 @@ struct MgRestrictEpi
             if ((kcur & 1) || kcur == g.n[2] - 1) {
 #pragma unroll
                 for (int p = 0; p < tile3::RY / 2; ++p) {
-                    if (even && c.valid[2 * p][0]) {
+                    if (lead && c.valid[2 * p][0]) {
                         const int gj = c.j0 + c.w * tile3::RY + 2 * p;
                         bc[pm.base + (gi >> 1) + pm.sy * (gj >> 1) + pm.sz * (kcur >> 1)] =
                             sum[p];
@@ -308,7 +366,7 @@ struct MgRestrictEpi
     }
 };
 
-template <class T>
+template <class T, int CPL = 1, bool PX = false>
 __global__ __launch_bounds__(256) void k_mg_resrestrict3(Geo g, Tile3Grid tg, MgParentMap pm,
                                                          const T* __restrict__ x,
                                                          const T* __restrict__ b,
@@ -318,9 +376,9 @@ __global__ __launch_bounds__(256) void k_mg_resrestrict3(Geo g, Tile3Grid tg, Mg
     using namespace tile3;
     if (st->done)
         return;
-    __shared__ T pl[2][TY + 2][prow<1>()];
-    MgRestrictEpi<T> epi{g, b, bc, pm};
-    tile3_march<T, 0, 1>(g, tg, x, nullptr, false, (T)0, pl, epi);
+    __shared__ __align__(16) T pl[2][TY + 2][prow<CPL>()];
+    MgRestrictEpi<T, CPL> epi{g, b, bc, pm};
+    tile3_march<T, 0, CPL, PX>(g, tg, x, nullptr, false, (T)0, pl, epi);
 }
 
 // children of parent index P along an axis of n cells
@@ -812,9 +870,10 @@ void launch_mg0_smooth(const Geo& g, const MgTab& wd0, int src, bool rz, const d
 // the tile grid of the tiled multigrid kernels: an even z chunk
 // (shorter on small levels, whose few tiles would otherwise leave most CUs
 // idle while each block marches its planes one after the other)
+template <int CPL = 1>
 static Tile3Grid mg_tile_grid(const Geo& g)
 {
-    Tile3Grid t = tile3_grid<1>(g);
+    Tile3Grid t = tile3_grid<CPL>(g);
     t.kz += t.kz & 1;
     while (t.kz > 2 && t.tiles_x * t.tiles_y * ((g.n[2] + t.kz - 1) / t.kz) < 1024)
         t.kz = (t.kz / 2 + 1) & ~1;
@@ -857,33 +916,53 @@ void launch_mg_smooth3(const Geo& g, const Geo& gw, int src, bool rz, const doub
 #undef CF_MG_SMOOTH3
 }
 
+// The FP32 level 0 marches two cells per lane (8-byte FP32 accesses, as the
+// FP32 PCG tiles), with the periodic-x variant on a periodic x axis; the
+// sweep that reads the FP64 r and reduces r.z keeps one cell per lane
+// (256^3, rocprof: 54.1 against 56.2 us; the others gain 1.4-3.7 us).
+template <bool PX>
+static void mg_smooth3_f32(const Geo& g, const Geo& gw, int src, bool rz, const float* xi,
+                           const float* b32, const double* r64, const Tile3Parent& par,
+                           float* xo, PcgState* st, RedBuf rb, hipStream_t s)
+{
+    const dim3 blk(kBlock);
+    if (rz) {
+        const Tile3Grid tg = mg_tile_grid<1>(g);
+        const dim3 grid((tg.work + 7) / 8 * 8);
+        if (src == MG_PLAIN)
+            hipLaunchKernelGGL((k_mg_smooth3<float, 0, true, 1, false>), grid, blk, 0, s, g, gw,
+                               tg, xi, r64, par, xo, st, rb);
+        else
+            hipLaunchKernelGGL((k_mg_smooth3<float, 2, true, 1, false>), grid, blk, 0, s, g, gw,
+                               tg, xi, r64, par, xo, st, rb);
+        return;
+    }
+    const Tile3Grid tg = mg_tile_grid<2>(g);
+    const dim3 grid((tg.work + 7) / 8 * 8);
+#define CF_MG_SMOOTH3F(M)                                                                 \
+    hipLaunchKernelGGL((k_mg_smooth3<float, M, false, 2, PX>), grid, blk, 0, s, g, gw, tg, xi, \
+                       b32, par, xo, st, rb)
+    if (src == MG_FIRST)
+        CF_MG_SMOOTH3F(1);
+    else if (src == MG_PLAIN)
+        CF_MG_SMOOTH3F(0);
+    else
+        CF_MG_SMOOTH3F(2);
+#undef CF_MG_SMOOTH3F
+}
+
 void launch_mg_smooth3_f32(const Geo& g, const Geo& gw, int src, bool rz, const float* xi,
                            const float* b32, const double* r64, const double* xc,
                            MgParentMap pm, double alpha, float* xo, PcgState* st, RedBuf rb,
                            hipStream_t s)
 {
-    const Tile3Grid tg = mg_tile_grid(g);
-    const dim3 grid((tg.work + 7) / 8 * 8), blk(kBlock);
-    const Tile3Parent par{xc, pm.base, pm.sy, pm.sz, alpha};
     if (rz && src == MG_FIRST)
         throw CfError(CF_EINVAL, "the FP32 first sweep does not reduce r.z");
-#define CF_MG_SMOOTH3F(M, RZ, B, XO)                                                        \
-    hipLaunchKernelGGL((k_mg_smooth3<float, M, RZ>), grid, blk, 0, s, g, gw, tg, xi, B, par, \
-                       XO, st, rb)
-    if (src == MG_FIRST) {
-        CF_MG_SMOOTH3F(1, false, b32, xo);
-    } else if (src == MG_PLAIN) {
-        if (rz)
-            CF_MG_SMOOTH3F(0, true, r64, xo);
-        else
-            CF_MG_SMOOTH3F(0, false, b32, xo);
-    } else {
-        if (rz)
-            CF_MG_SMOOTH3F(2, true, r64, xo);
-        else
-            CF_MG_SMOOTH3F(2, false, b32, xo);
-    }
-#undef CF_MG_SMOOTH3F
+    const Tile3Parent par{xc, pm.base, pm.sy, pm.sz, alpha};
+    if (g.periodic[0])
+        mg_smooth3_f32<true>(g, gw, src, rz, xi, b32, r64, par, xo, st, rb, s);
+    else
+        mg_smooth3_f32<false>(g, gw, src, rz, xi, b32, r64, par, xo, st, rb, s);
 }
 
 void launch_mg_resrestrict3(const Geo& g, const double* x, const double* b, double* bc,
@@ -897,9 +976,14 @@ void launch_mg_resrestrict3(const Geo& g, const double* x, const double* b, doub
 void launch_mg_resrestrict3_f32(const Geo& g, const float* x, const float* b, double* bc,
                                 MgParentMap pm, const PcgState* st, hipStream_t s)
 {
-    const Tile3Grid tg = mg_tile_grid(g);
-    hipLaunchKernelGGL(k_mg_resrestrict3<float>, dim3((tg.work + 7) / 8 * 8), dim3(kBlock), 0, s,
-                       g, tg, pm, x, b, bc, st);
+    const Tile3Grid tg = mg_tile_grid<2>(g);
+    const dim3 grid((tg.work + 7) / 8 * 8), blk(kBlock);
+    if (g.periodic[0])
+        hipLaunchKernelGGL((k_mg_resrestrict3<float, 2, true>), grid, blk, 0, s, g, tg, pm, x, b,
+                           bc, st);
+    else
+        hipLaunchKernelGGL((k_mg_resrestrict3<float, 2, false>), grid, blk, 0, s, g, tg, pm, x,
+                           b, bc, st);
 }
 
 void launch_mg0_first_f32(const Geo& g, const MgTab& wd0, const float* b, float* x,

@@ -413,7 +413,6 @@ __device__ __forceinline__ void tile3_load(const Tile3Ctx<CPL>& c, const Plane3M
 {
     static_assert(std::is_same<TA, T>::value || CPL == 1, "a widened operand: one cell per lane");
     constexpr bool DIR = MODE == 1;
-    static_assert(MODE != 2 || CPL == 1, "the prolongation mode runs one cell per lane");
 #pragma unroll
     for (int s = 0; s < tile3::NS; ++s) {
         const bool ok = c.colb[s] >= 0 && m.off >= 0;
@@ -422,7 +421,7 @@ __device__ __forceinline__ void tile3_load(const Tile3Ctx<CPL>& c, const Plane3M
         // lane's group of CPL cells
         const bool single = CPL > 1 && s == tile3::RY && c.hkind == 2;
         const bool split = CPL > 1 && (s < tile3::RY ? c.split : c.hsplit);
-        if constexpr (MODE == 2) {
+        if constexpr (MODE == 2 && CPL == 1) {
             const int64_t pat = par.base + (c.sgi[s] >> 1) + par.sy * (c.sgj[s] >> 1) +
                                 par.sz * (m.zc >> 1);
             P.a[s][0] = (T)load_any<TA>(va, ok, at, c.safe);
@@ -464,6 +463,25 @@ __device__ __forceinline__ void tile3_load(const Tile3Ctx<CPL>& c, const Plane3M
                     const T wb = vb[aw];
                     P.b[s][1] = split ? wb : P.b[s][1];
                 }
+            }
+            if constexpr (MODE == 2) {
+                // the parents of the slot's cells (the parent level is FP64;
+                // an FP32 level rounds its values): the second cell is the
+                // row's first (split), or a zero ghost past a non-periodic
+                // row's end, or absent (an x-halo slot)
+                // A pair starts at an even x, so both cells have one parent
+                // unless the second is the wrapped first cell (PX).
+                const int x0 = c.sgi[s];
+                const bool ghost1 = single || (!split && x0 + 1 > c.wrapback);
+                const int64_t prow =
+                    par.base + par.sy * (c.sgj[s] >> 1) + par.sz * (m.zc >> 1);
+                P.b[s][0] = (T)load_any<double>(par.xc, ok, prow + (x0 >> 1), par.base);
+                T b1 = P.b[s][0];
+                if constexpr (PX) {
+                    const T bw = (T)load_any<double>(par.xc, ok && split, prow, par.base);
+                    b1 = split ? bw : b1;
+                }
+                P.b[s][1] = ghost1 ? (T)0 : b1;
             }
         }
     }
@@ -716,6 +734,8 @@ __device__ __forceinline__ void tile3_march(
             const Plane3Meta mnn = plane3_meta(g, ahead2(k));
             tile3_load<T, DIR, CPL, PX, TA>(c, mnn, va, vb, need_b, par, P1);
             tile3_finish<T, DIR, CPL>(g, c, mn, P0, first, beta, pl[cur ^ 1], pp);
+            if constexpr (epi_takes_raw<E>::value)
+                epi.template take_raw<1>(P0);
             epi.template prefetch<1>(c, mn, k + D);
 #pragma unroll
             for (int q = 0; q < RY; ++q) {
