@@ -236,11 +236,11 @@ struct MgSweepEpi
 // Jacobi table), 0 = plain, 2 = after the coarse correction.  CPL, PX: the
 // tile shape (two cells per lane for an FP32 level, with its periodic-x
 // variant).
-template <class T, int MODE, bool RZ, int CPL = 1, bool PX = false>
+template <class T, int MODE, bool RZ, int CPL = 1, bool PX = false, class TP = double>
 __global__ __launch_bounds__(256) void k_mg_smooth3(Geo g, Geo gw, Tile3Grid tg,
                                                     const T* __restrict__ xi,
                                                     const MgRhsT<T, RZ>* __restrict__ b,
-                                                    Tile3Parent par, T* __restrict__ xo,
+                                                    Tile3ParentT<TP> par, T* __restrict__ xo,
                                                     PcgState* st, RedBuf rb)
 {
     using namespace tile3;
@@ -269,12 +269,14 @@ __global__ __launch_bounds__(256) void k_mg_smooth3(Geo g, Geo gw, Tile3Grid tg,
 // (children in x, y order), and after the second plane of a z pair (or the
 // last plane) writes the parent.  rotate() runs on every lane of an active
 // block, so the lane exchange is not under divergent control.
-template <class T, int CPL = 1>
+// TO: the parent level's type (the sums, in FP64, are rounded to an FP32
+// parent level).
+template <class T, int CPL = 1, class TO = double>
 struct MgRestrictEpi
 {
     const Geo& g;
     const T* __restrict__ b;
-    double* __restrict__ bc;
+    TO* __restrict__ bc;
     MgParentMap pm;
     const Tile3Ctx<CPL>* ctx = nullptr;
     // b of a plane, alternating with the march's steps; the residuals of
@@ -356,7 +358,7 @@ struct MgRestrictEpi
                     if (lead && c.valid[2 * p][0]) {
                         const int gj = c.j0 + c.w * tile3::RY + 2 * p;
                         bc[pm.base + (gi >> 1) + pm.sy * (gj >> 1) + pm.sz * (kcur >> 1)] =
-                            sum[p];
+                            (TO)sum[p];
                     }
                     sum[p] = 0.0;
                 }
@@ -366,18 +368,18 @@ struct MgRestrictEpi
     }
 };
 
-template <class T, int CPL = 1, bool PX = false>
+template <class T, int CPL = 1, bool PX = false, class TO = double>
 __global__ __launch_bounds__(256) void k_mg_resrestrict3(Geo g, Tile3Grid tg, MgParentMap pm,
                                                          const T* __restrict__ x,
                                                          const T* __restrict__ b,
-                                                         double* __restrict__ bc,
+                                                         TO* __restrict__ bc,
                                                          const PcgState* st)
 {
     using namespace tile3;
     if (st->done)
         return;
     __shared__ __align__(16) T pl[2][TY + 2][prow<CPL>()];
-    MgRestrictEpi<T, CPL> epi{g, b, bc, pm};
+    MgRestrictEpi<T, CPL, TO> epi{g, b, bc, pm};
     tile3_march<T, 0, CPL, PX>(g, tg, x, nullptr, false, (T)0, pl, epi);
 }
 
@@ -919,10 +921,12 @@ void launch_mg_smooth3(const Geo& g, const Geo& gw, int src, bool rz, const doub
 // The FP32 level 0 marches two cells per lane (8-byte FP32 accesses, as the
 // FP32 PCG tiles), with the periodic-x variant on a periodic x axis; the
 // sweep that reads the FP64 r and reduces r.z keeps one cell per lane
-// (256^3, rocprof: 54.1 against 56.2 us; the others gain 1.4-3.7 us).
-template <bool PX>
+// (256^3, rocprof: 54.1 against 56.2 us; the others gain 1.4-3.7 us).  An
+// FP32 level 1 marches one cell per lane (at 128^3 two would leave 512
+// blocks), with an FP64 (level 2) or FP32 (level 1 for level 0) parent.
+template <int CPL, bool PX, class TP>
 static void mg_smooth3_f32(const Geo& g, const Geo& gw, int src, bool rz, const float* xi,
-                           const float* b32, const double* r64, const Tile3Parent& par,
+                           const float* b32, const double* r64, const Tile3ParentT<TP>& par,
                            float* xo, PcgState* st, RedBuf rb, hipStream_t s)
 {
     const dim3 blk(kBlock);
@@ -930,18 +934,18 @@ static void mg_smooth3_f32(const Geo& g, const Geo& gw, int src, bool rz, const 
         const Tile3Grid tg = mg_tile_grid<1>(g);
         const dim3 grid((tg.work + 7) / 8 * 8);
         if (src == MG_PLAIN)
-            hipLaunchKernelGGL((k_mg_smooth3<float, 0, true, 1, false>), grid, blk, 0, s, g, gw,
-                               tg, xi, r64, par, xo, st, rb);
+            hipLaunchKernelGGL((k_mg_smooth3<float, 0, true, 1, false, TP>), grid, blk, 0, s, g,
+                               gw, tg, xi, r64, par, xo, st, rb);
         else
-            hipLaunchKernelGGL((k_mg_smooth3<float, 2, true, 1, false>), grid, blk, 0, s, g, gw,
-                               tg, xi, r64, par, xo, st, rb);
+            hipLaunchKernelGGL((k_mg_smooth3<float, 2, true, 1, false, TP>), grid, blk, 0, s, g,
+                               gw, tg, xi, r64, par, xo, st, rb);
         return;
     }
-    const Tile3Grid tg = mg_tile_grid<2>(g);
+    const Tile3Grid tg = mg_tile_grid<CPL>(g);
     const dim3 grid((tg.work + 7) / 8 * 8);
-#define CF_MG_SMOOTH3F(M)                                                                 \
-    hipLaunchKernelGGL((k_mg_smooth3<float, M, false, 2, PX>), grid, blk, 0, s, g, gw, tg, xi, \
-                       b32, par, xo, st, rb)
+#define CF_MG_SMOOTH3F(M)                                                                     \
+    hipLaunchKernelGGL((k_mg_smooth3<float, M, false, CPL, PX, TP>), grid, blk, 0, s, g, gw, tg, \
+                       xi, b32, par, xo, st, rb)
     if (src == MG_FIRST)
         CF_MG_SMOOTH3F(1);
     else if (src == MG_PLAIN)
@@ -953,16 +957,30 @@ static void mg_smooth3_f32(const Geo& g, const Geo& gw, int src, bool rz, const 
 
 void launch_mg_smooth3_f32(const Geo& g, const Geo& gw, int src, bool rz, const float* xi,
                            const float* b32, const double* r64, const double* xc,
-                           MgParentMap pm, double alpha, float* xo, PcgState* st, RedBuf rb,
-                           hipStream_t s)
+                           const float* xc32, MgParentMap pm, double alpha, float* xo,
+                           PcgState* st, RedBuf rb, hipStream_t s, int cpl)
 {
     if (rz && src == MG_FIRST)
         throw CfError(CF_EINVAL, "the FP32 first sweep does not reduce r.z");
+    if (cpl != 1 && cpl != 2)
+        throw CfError(CF_EINVAL, "cells per lane of an FP32 multigrid level: 1 or 2");
+    if (xc32) {
+        const Tile3ParentT<float> par{xc32, pm.base, pm.sy, pm.sz, alpha};
+        if (cpl == 1)
+            mg_smooth3_f32<1, false>(g, gw, src, rz, xi, b32, r64, par, xo, st, rb, s);
+        else if (g.periodic[0])
+            mg_smooth3_f32<2, true>(g, gw, src, rz, xi, b32, r64, par, xo, st, rb, s);
+        else
+            mg_smooth3_f32<2, false>(g, gw, src, rz, xi, b32, r64, par, xo, st, rb, s);
+        return;
+    }
     const Tile3Parent par{xc, pm.base, pm.sy, pm.sz, alpha};
-    if (g.periodic[0])
-        mg_smooth3_f32<true>(g, gw, src, rz, xi, b32, r64, par, xo, st, rb, s);
+    if (cpl == 1)
+        mg_smooth3_f32<1, false>(g, gw, src, rz, xi, b32, r64, par, xo, st, rb, s);
+    else if (g.periodic[0])
+        mg_smooth3_f32<2, true>(g, gw, src, rz, xi, b32, r64, par, xo, st, rb, s);
     else
-        mg_smooth3_f32<false>(g, gw, src, rz, xi, b32, r64, par, xo, st, rb, s);
+        mg_smooth3_f32<2, false>(g, gw, src, rz, xi, b32, r64, par, xo, st, rb, s);
 }
 
 void launch_mg_resrestrict3(const Geo& g, const double* x, const double* b, double* bc,
@@ -973,17 +991,36 @@ void launch_mg_resrestrict3(const Geo& g, const double* x, const double* b, doub
                        s, g, tg, pm, x, b, bc, st);
 }
 
-void launch_mg_resrestrict3_f32(const Geo& g, const float* x, const float* b, double* bc,
+template <int CPL, bool PX, class TO>
+static void mg_resrestrict3_f32(const Geo& g, const float* x, const float* b, TO* bc,
                                 MgParentMap pm, const PcgState* st, hipStream_t s)
 {
-    const Tile3Grid tg = mg_tile_grid<2>(g);
-    const dim3 grid((tg.work + 7) / 8 * 8), blk(kBlock);
-    if (g.periodic[0])
-        hipLaunchKernelGGL((k_mg_resrestrict3<float, 2, true>), grid, blk, 0, s, g, tg, pm, x, b,
-                           bc, st);
+    const Tile3Grid tg = mg_tile_grid<CPL>(g);
+    hipLaunchKernelGGL((k_mg_resrestrict3<float, CPL, PX, TO>), dim3((tg.work + 7) / 8 * 8),
+                       dim3(kBlock), 0, s, g, tg, pm, x, b, bc, st);
+}
+
+void launch_mg_resrestrict3_f32(const Geo& g, const float* x, const float* b, double* bc,
+                                float* bc32, MgParentMap pm, const PcgState* st, hipStream_t s,
+                                int cpl)
+{
+    if (cpl != 1 && cpl != 2)
+        throw CfError(CF_EINVAL, "cells per lane of an FP32 multigrid level: 1 or 2");
+    if (bc32) {
+        if (cpl == 1)
+            mg_resrestrict3_f32<1, false>(g, x, b, bc32, pm, st, s);
+        else if (g.periodic[0])
+            mg_resrestrict3_f32<2, true>(g, x, b, bc32, pm, st, s);
+        else
+            mg_resrestrict3_f32<2, false>(g, x, b, bc32, pm, st, s);
+        return;
+    }
+    if (cpl == 1)
+        mg_resrestrict3_f32<1, false>(g, x, b, bc, pm, st, s);
+    else if (g.periodic[0])
+        mg_resrestrict3_f32<2, true>(g, x, b, bc, pm, st, s);
     else
-        hipLaunchKernelGGL((k_mg_resrestrict3<float, 2, false>), grid, blk, 0, s, g, tg, pm, x,
-                           b, bc, st);
+        mg_resrestrict3_f32<2, false>(g, x, b, bc, pm, st, s);
 }
 
 void launch_mg0_first_f32(const Geo& g, const MgTab& wd0, const float* b, float* x,

@@ -112,8 +112,8 @@ void launch_mg_smooth3(const Geo& g, const Geo& gw, int src, bool rz, const doub
 void launch_mg_resrestrict3(const Geo& g, const double* x, const double* b, double* bc,
                             MgParentMap pm, const PcgState* st, hipStream_t s);
 
-// FP32 level 0 (3-D, marched): the same sweeps and restriction on FP32
-// vectors.  b32 = (float) r (launch_mg_cast32, or the PCG update); a sweep
+// FP32 level 0 (3-D, marched), and level 1 when it is a marched tiled level:
+// the same sweeps and restriction on FP32 vectors.  b32 = (float) r (launch_mg_cast32, or the PCG update); a sweep
 // with rz reads r64 instead of b32 (its sweep uses (float) r64) and reduces
 // r.z with the FP64 r; the restriction sums the FP32 residuals as doubles
 // into the FP64 level 1; the prolongation rounds the FP64 parent to FP32.
@@ -121,12 +121,16 @@ void launch_mg_cast32(const Geo& g, const double* r, float* b32, const PcgState*
                       hipStream_t s);
 void launch_mg0_first_f32(const Geo& g, const MgTab& wd0, const float* b, float* x,
                           const PcgState* st, hipStream_t s);
+// xc32 (non-null): an FP32 parent level instead of xc; cpl: cells per lane
+// (2 on level 0, 1 on an FP32 level 1).
 void launch_mg_smooth3_f32(const Geo& g, const Geo& gw, int src, bool rz, const float* xi,
                            const float* b32, const double* r64, const double* xc,
-                           MgParentMap pm, double alpha, float* xo, PcgState* st, RedBuf rb,
-                           hipStream_t s);
+                           const float* xc32, MgParentMap pm, double alpha, float* xo,
+                           PcgState* st, RedBuf rb, hipStream_t s, int cpl);
+// bc32 (non-null): the sums rounded into an FP32 level 1 instead of bc.
 void launch_mg_resrestrict3_f32(const Geo& g, const float* x, const float* b, double* bc,
-                                MgParentMap pm, const PcgState* st, hipStream_t s);
+                                float* bc32, MgParentMap pm, const PcgState* st, hipStream_t s,
+                                int cpl);
 
 // Compact levels.
 void launch_mg_first(const MgLev& L, const double* b, double* x, const PcgState* st,
@@ -173,9 +177,11 @@ class Multigrid
     void smooth(int l, int src, bool rz, const double* xi, const double* b, double* xo,
                 PcgState* st, RedBuf rb);
     void resrestrict(int l, const double* x, const double* b, PcgState* st);
-    // one FP32 level-0 sweep (launch_mg_smooth3_f32)
-    void smooth32(int src, bool rz, const float* xi, const double* r, float* xo, PcgState* st,
-                  RedBuf rb);
+    // one FP32 sweep on level l (launch_mg_smooth3_f32)
+    void smooth32(int l, int src, bool rz, const float* xi, const float* b32, const double* r,
+                  float* xo, const float* xc32, PcgState* st, RedBuf rb);
+    // level 1 runs in FP32 too (with level 0): it is a marched tile level
+    bool f32_level1() const;
     bool marched(int l) const;
     const Geo& _g;
     hipStream_t _s;

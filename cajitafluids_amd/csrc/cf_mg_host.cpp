@@ -33,7 +33,8 @@ namespace cfmi
 struct Multigrid::Buffers
 {
     DevArray<double> z[2];
-    DevArray<float> f[3]; // FP32 level 0: x, t, b32 (allocated on first use)
+    DevArray<float> f[3];  // FP32 level 0: x, t, b32 (allocated on first use)
+    DevArray<float> f1[3]; // FP32 level 1: x, t, b (when f32_level1())
     std::vector<std::unique_ptr<DevArray<double>>> vec;
     std::vector<double*> X, T, B, res;
     MgTail* tail = nullptr; // device copy
@@ -75,6 +76,10 @@ void tiled_level_geo(const Geo& g, const MgLev& L, int l, double omega, Geo& lg,
         lg.dinv[k] = K > 0 ? 1.0 / lg.diag[k] : 0.0;
         lgw.diag[k] = lg.diag[k];
         lgw.dinv[k] = K > 0 ? omega / lg.diag[k] : 0.0;
+        // an FP32 level: the FP64 coefficients rounded
+        lg.diag32[k] = lgw.diag32[k] = (float)lg.diag[k];
+        lg.dinv32[k] = (float)lg.dinv[k];
+        lgw.dinv32[k] = (float)lgw.dinv[k];
     }
 }
 
@@ -276,12 +281,19 @@ void Multigrid::resrestrict(int l, const double* x, const double* b, PcgState* s
         launch_mg_resrestrict(_lev[l], _lev[l + 1], x, b, bc, st, _s);
 }
 
-void Multigrid::smooth32(int src, bool rz, const float* xi, const double* r, float* xo,
-                         PcgState* st, RedBuf rb)
+// one FP32 sweep on level l (0, or 1 when f32_level1()); the prolongation
+// source is level l + 1's cycle result (xc32 when that level is FP32)
+void Multigrid::smooth32(int l, int src, bool rz, const float* xi, const float* b32,
+                         const double* r, float* xo, const float* xc32, PcgState* st, RedBuf rb)
 {
     Buffers& B = *_buf;
-    launch_mg_smooth3_f32(_lg[0], _lgw[0], src, rz, xi, B.f[2].ptr, r, B.res[1], _pm[0],
-                          _lev[0].alpha, xo, st, rb, _s);
+    launch_mg_smooth3_f32(_lg[l], _lgw[l], src, rz, xi, b32, r, xc32 ? nullptr : B.res[l + 1],
+                          xc32, _pm[l], _lev[0].alpha, xo, st, rb, _s, l == 0 ? 2 : 1);
+}
+
+bool Multigrid::f32_level1() const
+{
+    return f32_supported() && _nt >= 2 && marched(1) && _tail0 > 1 && _lev.size() >= 3;
 }
 
 float* Multigrid::b32() const { return _buf->f[2].ptr; }
@@ -323,27 +335,48 @@ const void* Multigrid::cycle(const double* r, PcgState* st, RedBuf rb, bool f32,
         launch_mgpcg_rz(_g, r, x, st, rb, _s);
         return x;
     }
-    // FP32 level 0: b32 = (float) r, pre-smoothing and restriction into the
-    // FP64 level 1 (the up phase below ends level 0 in FP32 too)
-    const int l0 = f32 ? 1 : 0; // first level of the FP64 loops
-    float* x32 = nullptr;
+    // FP32 level 0 (and level 1 when it is a marched tile level): b32 =
+    // (float) r, pre-smoothing and restriction down to the first FP64 level
+    // (the up phase below ends these levels in FP32 too)
+    const bool f1 = f32 && f32_level1();
+    const int l0 = f32 ? (f1 ? 2 : 1) : 0; // first level of the FP64 loops
+    float *x32 = nullptr, *x1f = nullptr;
     if (f32) {
-        for (int i = 0; i < 3; ++i)
+        for (int i = 0; i < 3; ++i) {
             if (!B.f[i].ptr)
                 B.f[i].alloc(_g.total, _s);
+            if (f1 && !B.f1[i].ptr)
+                B.f1[i].alloc(_lg[1].total, _s);
+        }
         float *x = B.f[0].ptr, *t = B.f[1].ptr;
+        float* b0 = B.f[2].ptr;
         if (!b32_ready)
-            launch_mg_cast32(_g, r, B.f[2].ptr, st, _s);
+            launch_mg_cast32(_g, r, b0, st, _s);
         if (_nu == 1)
-            launch_mg0_first_f32(_g, _wt[0], B.f[2].ptr, x, st, _s);
+            launch_mg0_first_f32(_g, _wt[0], b0, x, st, _s);
         else
-            smooth32(MG_FIRST, false, nullptr, r, x, st, rb);
+            smooth32(0, MG_FIRST, false, nullptr, b0, r, x, nullptr, st, rb);
         for (int it = 2; it < _nu; ++it) {
-            smooth32(MG_PLAIN, false, x, r, t, st, rb);
+            smooth32(0, MG_PLAIN, false, x, b0, r, t, nullptr, st, rb);
             std::swap(x, t);
         }
-        launch_mg_resrestrict3_f32(_lg[0], x, B.f[2].ptr, B.B[1], _pm[0], st, _s);
+        launch_mg_resrestrict3_f32(_lg[0], x, b0, B.B[1], f1 ? B.f1[2].ptr : nullptr, _pm[0], st,
+                                   _s, 2);
         x32 = x;
+        if (f1) {
+            float *y = B.f1[0].ptr, *u = B.f1[1].ptr;
+            const float* b1 = B.f1[2].ptr;
+            if (_nu == 1)
+                launch_mg0_first_f32(_lg[1], _wt[1], b1, y, st, _s);
+            else
+                smooth32(1, MG_FIRST, false, nullptr, b1, nullptr, y, nullptr, st, rb);
+            for (int it = 2; it < _nu; ++it) {
+                smooth32(1, MG_PLAIN, false, y, b1, nullptr, u, nullptr, st, rb);
+                std::swap(y, u);
+            }
+            launch_mg_resrestrict3_f32(_lg[1], y, b1, B.B[2], nullptr, _pm[1], st, _s, 1);
+            x1f = y;
+        }
     }
     // down: pre-smoothing from zero and restriction
     for (int l = l0; l < _tail0; ++l) {
@@ -377,14 +410,28 @@ const void* Multigrid::cycle(const double* r, PcgState* st, RedBuf rb, bool f32,
         }
     }
     if (f32) {
+        const float* xc32 = nullptr; // level 0's parent when level 1 is FP32
+        if (f1) {
+            float* y = x1f;
+            float* u = y == B.f1[0].ptr ? B.f1[1].ptr : B.f1[0].ptr;
+            const float* b1 = B.f1[2].ptr;
+            smooth32(1, MG_PROLONG, false, y, b1, nullptr, u, nullptr, st, rb);
+            std::swap(y, u);
+            for (int it = 1; it < _nu; ++it) {
+                smooth32(1, MG_PLAIN, false, y, b1, nullptr, u, nullptr, st, rb);
+                std::swap(y, u);
+            }
+            xc32 = y;
+        }
         // level 0 in FP32: the last sweep writes z and reduces r.z from the
         // FP64 r; z stays in FP32 (the direction kernel widens it)
         float* xf = x32;
         float* tf = xf == B.f[0].ptr ? B.f[1].ptr : B.f[0].ptr;
-        smooth32(MG_PROLONG, _nu == 1, xf, r, tf, st, rb);
+        const float* b0 = B.f[2].ptr;
+        smooth32(0, MG_PROLONG, _nu == 1, xf, b0, r, tf, xc32, st, rb);
         std::swap(xf, tf);
         for (int it = 1; it < _nu; ++it) {
-            smooth32(MG_PLAIN, it == _nu - 1, xf, r, tf, st, rb);
+            smooth32(0, MG_PLAIN, it == _nu - 1, xf, b0, r, tf, xc32, st, rb);
             std::swap(xf, tf);
         }
         return xf;

@@ -80,12 +80,14 @@ struct Tile3Grid
 // Parent grid of MODE 2: value(i, j, k) = fma(alpha, xc[base + i/2 + sy j/2 +
 // sz k/2], a(i, j, k)) (the parent array compact or padded); the march
 // receives alpha as its `beta`.
-struct Tile3Parent
+template <class TP>
+struct Tile3ParentT
 {
-    const double* xc;
+    const TP* xc; // the parent level's values (FP64, or an FP32 level's)
     int64_t base, sy, sz;
     double alpha;
 };
+using Tile3Parent = Tile3ParentT<double>;
 
 // An epilogue that declares `static constexpr bool kRaw = true` receives the
 // raw loaded operand of its own rows (epi.take_raw(plane)) just before each
@@ -405,11 +407,12 @@ struct Plane3
 
 // TA: the element type of va (an FP32 operand widened to T on load, one
 // cell per lane only).
-template <class T, int MODE, int CPL, bool PX = false, class TA = T>
+template <class T, int MODE, int CPL, bool PX = false, class TA = T, class TP = double>
 __device__ __forceinline__ void tile3_load(const Tile3Ctx<CPL>& c, const Plane3Meta& m,
                                            const TA* __restrict__ va,
                                            const T* __restrict__ vb, bool need_b,
-                                           const Tile3Parent& par, Plane3<T, MODE, CPL>& P)
+                                           const Tile3ParentT<TP>& par,
+                                           Plane3<T, MODE, CPL>& P)
 {
     static_assert(std::is_same<TA, T>::value || CPL == 1, "a widened operand: one cell per lane");
     constexpr bool DIR = MODE == 1;
@@ -425,8 +428,8 @@ __device__ __forceinline__ void tile3_load(const Tile3Ctx<CPL>& c, const Plane3M
             const int64_t pat = par.base + (c.sgi[s] >> 1) + par.sy * (c.sgj[s] >> 1) +
                                 par.sz * (m.zc >> 1);
             P.a[s][0] = (T)load_any<TA>(va, ok, at, c.safe);
-            // the parent level is FP64; an FP32 level rounds its value
-            P.b[s][0] = (T)load_any<double>(par.xc, ok, pat, par.base);
+            // an FP32 level rounds an FP64 parent's value
+            P.b[s][0] = (T)load_any<TP>(par.xc, ok, pat, par.base);
             continue;
         } else if constexpr (CPL == 1) {
             // the direction's old value is read on the first iteration too
@@ -475,10 +478,10 @@ __device__ __forceinline__ void tile3_load(const Tile3Ctx<CPL>& c, const Plane3M
                 const bool ghost1 = single || (!split && x0 + 1 > c.wrapback);
                 const int64_t prow =
                     par.base + par.sy * (c.sgj[s] >> 1) + par.sz * (m.zc >> 1);
-                P.b[s][0] = (T)load_any<double>(par.xc, ok, prow + (x0 >> 1), par.base);
+                P.b[s][0] = (T)load_any<TP>(par.xc, ok, prow + (x0 >> 1), par.base);
                 T b1 = P.b[s][0];
                 if constexpr (PX) {
-                    const T bw = (T)load_any<double>(par.xc, ok && split, prow, par.base);
+                    const T bw = (T)load_any<TP>(par.xc, ok && split, prow, par.base);
                     b1 = split ? bw : b1;
                 }
                 P.b[s][1] = ghost1 ? (T)0 : b1;
@@ -635,12 +638,12 @@ __device__ __forceinline__ T tile3_apply(const Geo& g, const Tile3Ctx<CPL>& c,
 // last, which are still in the Infinity Cache.  The stencil sees the same
 // operands in the same order either way.
 template <class T, int DIR, int CPL, bool PX = false, bool REV = false, class TA = T,
-          class Epi>
+          class Epi, class TP = double>
 __device__ __forceinline__ void tile3_march(
     const Geo& g, const Tile3Grid& tg, const TA* __restrict__ va,
     const T* __restrict__ vb, bool first, T beta,
     T (*pl)[tile3::TY + 2][tile3::prow<CPL>()], Epi&& epi,
-    const Tile3Parent par = Tile3Parent{nullptr, 0, 0, 0, 0.0})
+    const Tile3ParentT<TP> par = Tile3ParentT<TP>{nullptr, 0, 0, 0, 0.0})
 {
     using namespace tile3;
     using E = std::decay_t<Epi>;
@@ -664,17 +667,17 @@ __device__ __forceinline__ void tile3_march(
     // prologue: the plane behind kbeg (own column only), kbeg into LDS, the
     // plane ahead in flight
     Plane3Meta mm = plane3_meta(g, kbeg - D);
-    tile3_load<T, DIR, CPL, PX, TA>(c, mm, va, vb, need_b, par, P0);
+    tile3_load<T, DIR, CPL, PX, TA, TP>(c, mm, va, vb, need_b, par, P0);
 #pragma unroll
     for (int q = 0; q < RY; ++q)
 #pragma unroll
         for (int j = 0; j < CPL; ++j)
             pm[q][j] = tile3_value<T, DIR, CPL>(g, c, mm, P0, q, j, first, beta);
     Plane3Meta mc = plane3_meta(g, kbeg);
-    tile3_load<T, DIR, CPL, PX, TA>(c, mc, va, vb, need_b, par, P1);
+    tile3_load<T, DIR, CPL, PX, TA, TP>(c, mc, va, vb, need_b, par, P1);
     tile3_finish<T, DIR, CPL>(g, c, mc, P1, first, beta, pl[0], pc);
     Plane3Meta mn = plane3_meta(g, kbeg + D);
-    tile3_load<T, DIR, CPL, PX, TA>(c, mn, va, vb, need_b, par, P0);
+    tile3_load<T, DIR, CPL, PX, TA, TP>(c, mn, va, vb, need_b, par, P0);
     if constexpr (epi_takes_raw<E>::value)
         epi.template take_raw<0>(P1);
     epi.template prefetch<0>(c, mc, kbeg);
@@ -689,7 +692,7 @@ __device__ __forceinline__ void tile3_march(
         // the last step loads the plane past the chunk again (an L2 hit,
         // never used) rather than branching around the loads
         const Plane3Meta mnn = plane3_meta(g, ahead2(k));
-        tile3_load<T, DIR, CPL, PX, TA>(c, mnn, va, vb, need_b, par, Pl);
+        tile3_load<T, DIR, CPL, PX, TA, TP>(c, mnn, va, vb, need_b, par, Pl);
         tile3_finish<T, DIR, CPL>(g, c, mn, Pf, first, beta, pl[S ^ 1], pp);
         if constexpr (epi_takes_raw<E>::value)
             epi.template take_raw<S ^ 1>(Pf);
@@ -732,7 +735,7 @@ __device__ __forceinline__ void tile3_march(
         int cur = 0;
         for (int s = 0, k = kbeg; s < c.k1 - c.k0; ++s, k += D) {
             const Plane3Meta mnn = plane3_meta(g, ahead2(k));
-            tile3_load<T, DIR, CPL, PX, TA>(c, mnn, va, vb, need_b, par, P1);
+            tile3_load<T, DIR, CPL, PX, TA, TP>(c, mnn, va, vb, need_b, par, P1);
             tile3_finish<T, DIR, CPL>(g, c, mn, P0, first, beta, pl[cur ^ 1], pp);
             if constexpr (epi_takes_raw<E>::value)
                 epi.template take_raw<1>(P0);

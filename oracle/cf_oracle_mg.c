@@ -40,6 +40,13 @@
  *              fmaf((float) alpha, (float) x_parent, x); nu sweeps;
  *              z = (double) x.  The CG around it (residuals, dots, alpha,
  *              beta, x) stays FP64.
+ *              Level 1 runs in FP32 as well when it is a full-box level of
+ *              more than 32768 cells (every level-0 axis even, every level-1
+ *              axis > 2: the levels the device marches with its tiles):
+ *              the level-0 sums are rounded into it, its coefficients are
+ *              the FP64 ones rounded, its residuals are summed as doubles
+ *              into level 2, and level 0's prolongation reads its FP32
+ *              result.
  *
  * A-x order everywhere: diag x, then fma over -x, +x, -y, +y, -z, +z.
  */
@@ -68,6 +75,8 @@ struct cfo_mg {
     double* w0;        /* level-0 omega * M (ghosted) */
     /* FP32 level 0 (ghosted, allocated on first use) */
     float *x0f, *t0f, *b0f, *r0f, *w0f;
+    /* FP32 level 1 (compact), when mg_f32_level1 */
+    float *x1f, *t1f, *b1f, *r1f;
 };
 
 static long lidx(const cfo_mglev* L, int i, int j, int k)
@@ -115,6 +124,10 @@ void cfo_mg_free(struct cfo_state* s)
     free(mg->b0f);
     free(mg->r0f);
     free(mg->w0f);
+    free(mg->x1f);
+    free(mg->t1f);
+    free(mg->b1f);
+    free(mg->r1f);
     free(mg);
     s->mg = NULL;
 }
@@ -388,9 +401,32 @@ static int mg_f32_supported(const struct cfo_state* s)
     return s->dim == 3 && s->n[0] > 2 && s->n[1] > 2 && s->n[2] > 2 && s->mg->nlev >= 2;
 }
 
+/* Level 1 in FP32 too: a full-box level (every level-0 axis even, every
+ * level-1 axis coarsened further) of more than 32768 cells. */
+static int mg_f32_level1(const struct cfo_state* s)
+{
+    const struct cfo_mg* mg = s->mg;
+    if (mg->nlev < 3)
+        return 0;
+    const cfo_mglev* L1 = &mg->lev[1];
+    for (int d = 0; d < 3; ++d)
+        if (s->n[d] % 2 != 0 || !L1->coarsen[d])
+            return 0;
+    return L1->ncell > 32768;
+}
+
 static int mg_f32_alloc(struct cfo_state* s)
 {
     struct cfo_mg* mg = s->mg;
+    if (mg_f32_level1(s) && !mg->x1f) {
+        size_t n1 = (size_t)mg->lev[1].ncell;
+        mg->x1f = (float*)calloc(n1, sizeof(float));
+        mg->t1f = (float*)calloc(n1, sizeof(float));
+        mg->b1f = (float*)calloc(n1, sizeof(float));
+        mg->r1f = (float*)calloc(n1, sizeof(float));
+        if (!mg->x1f || !mg->t1f || !mg->b1f || !mg->r1f)
+            return CFO_ENOMEM;
+    }
     if (mg->x0f)
         return CFO_OK;
     long nc = s->size[0];
@@ -415,7 +451,87 @@ static void mg_sweep0_f32(struct cfo_state* s, float* xi, const float* b, float*
     CFO_FOR_OWNED_CELLS(s, xo[c] = fmaf(mg->w0f[c], b[c] - xo[c], xi[c]));
 }
 
-/* z = B r with level 0 in FP32 (levels >= 1 as mg_vcycle_level) */
+/* y = A_l x on a compact level in FP32: the FP64 coefficients rounded, the
+ * order of mg_apply */
+static void mg_apply_f32(const struct cfo_state* s, const cfo_mglev* L, const float* x,
+                         float* y)
+{
+    const int D = s->dim;
+    for (int k = 0; k < L->n[2]; ++k)
+        for (int j = 0; j < L->n[1]; ++j)
+            for (int i = 0; i < L->n[0]; ++i) {
+                int c[3] = {i, j, k};
+                long id = lidx(L, i, j, k);
+                float acc = (float)L->diag[id] * x[id];
+                for (int d = 0; d < D; ++d) {
+                    long m = lnb(s, L, c, d, -1), p = lnb(s, L, c, d, +1);
+                    float cm = m >= 0 ? (float)L->cp[d][m] : 0.0f;
+                    float cpl = p >= 0 ? (float)L->cp[d][id] : 0.0f;
+                    acc = fmaf(cm, m >= 0 ? x[m] : 0.0f, acc);
+                    acc = fmaf(cpl, p >= 0 ? x[p] : 0.0f, acc);
+                }
+                y[id] = acc;
+            }
+}
+
+/* one FP32 Jacobi sweep on a compact level: xo = fmaf(w, b - A xi, xi) */
+static void mg_sweep_f32(const struct cfo_state* s, const cfo_mglev* L, const float* xi,
+                         const float* b, float* xo)
+{
+    mg_apply_f32(s, L, xi, xo);
+    for (long id = 0; id < L->ncell; ++id)
+        xo[id] = fmaf((float)L->w[id], b[id] - xo[id], xi[id]);
+}
+
+/* the FP32 level 1 (mg_f32_level1): b1f is its right-hand side; on return
+ * x1f holds its cycle result */
+static void mg_vcycle1_f32(struct cfo_state* s)
+{
+    struct cfo_mg* mg = s->mg;
+    const cfo_mglev* L = &mg->lev[1];
+    const cfo_mglev* C = &mg->lev[2];
+    const int nu = s->c.mg_nu;
+    float *x = mg->x1f, *t = mg->t1f, *b = mg->b1f, *rf = mg->r1f;
+    for (long id = 0; id < L->ncell; ++id)
+        x[id] = (float)L->w[id] * b[id];
+    for (int it = 1; it < nu; ++it) {
+        mg_sweep_f32(s, L, x, b, t);
+        memcpy(x, t, (size_t)L->ncell * sizeof(float));
+    }
+    mg_apply_f32(s, L, x, rf);
+    for (long id = 0; id < L->ncell; ++id)
+        rf[id] = b[id] - rf[id];
+    for (int K = 0; K < C->n[2]; ++K)
+        for (int J = 0; J < C->n[1]; ++J)
+            for (int I = 0; I < C->n[0]; ++I) {
+                int Ic[3] = {I, J, K}, lo[3], hi[3];
+                for (int d = 0; d < 3; ++d) {
+                    lo[d] = 2 * Ic[d];
+                    hi[d] = 2 * Ic[d] + 1 < L->n[d] ? 2 * Ic[d] + 1 : 2 * Ic[d];
+                }
+                double sum = 0.0;
+                for (int k = lo[2]; k <= hi[2]; ++k)
+                    for (int j = lo[1]; j <= hi[1]; ++j)
+                        for (int i = lo[0]; i <= hi[0]; ++i)
+                            sum = sum + (double)rf[lidx(L, i, j, k)];
+                mg->lev[2].b[lidx(C, I, J, K)] = sum;
+            }
+    mg_vcycle_level(s, 2, NULL, NULL);
+    const float a = (float)s->c.mg_alpha;
+    for (int k = 0; k < L->n[2]; ++k)
+        for (int j = 0; j < L->n[1]; ++j)
+            for (int i = 0; i < L->n[0]; ++i) {
+                long fi = lidx(L, i, j, k);
+                x[fi] = fmaf(a, (float)mg->lev[2].x[lidx(C, i / 2, j / 2, k / 2)], x[fi]);
+            }
+    for (int it = 0; it < nu; ++it) {
+        mg_sweep_f32(s, L, x, b, t);
+        memcpy(x, t, (size_t)L->ncell * sizeof(float));
+    }
+}
+
+/* z = B r with level 0 in FP32 (levels >= 1 as mg_vcycle_level, or level 1
+ * in FP32 too) */
 static void mg_vcycle0_f32(struct cfo_state* s, double* z, const double* r)
 {
     struct cfo_mg* mg = s->mg;
@@ -432,7 +548,9 @@ static void mg_vcycle0_f32(struct cfo_state* s, double* z, const double* r)
         cfo_fill_ghosts_f32(s, x);
     cfo_spmv32(s, x, rf);
     CFO_FOR_OWNED_CELLS(s, rf[c] = b[c] - rf[c]);
-    /* restriction: the children's FP32 residuals summed as doubles */
+    const int f1 = mg_f32_level1(s);
+    /* restriction: the children's FP32 residuals summed as doubles (and
+     * rounded into an FP32 level 1) */
     {
         const cfo_mglev* F = &mg->lev[0];
         const cfo_mglev* C = &mg->lev[1];
@@ -449,11 +567,17 @@ static void mg_vcycle0_f32(struct cfo_state* s, double* z, const double* r)
                         for (int j = lo[1]; j <= hi[1]; ++j)
                             for (int i = lo[0]; i <= hi[0]; ++i)
                                 sum = sum + (double)rf[g0(s, i, j, k)];
-                    mg->lev[1].b[lidx(C, I, J, K)] = sum;
+                    if (f1)
+                        mg->b1f[lidx(C, I, J, K)] = (float)sum;
+                    else
+                        mg->lev[1].b[lidx(C, I, J, K)] = sum;
                 }
     }
-    mg_vcycle_level(s, 1, NULL, NULL);
-    /* prolongation in FP32 from the FP64 level-1 result */
+    if (f1)
+        mg_vcycle1_f32(s);
+    else
+        mg_vcycle_level(s, 1, NULL, NULL);
+    /* prolongation in FP32 from the level-1 result (FP64, or FP32) */
     {
         const cfo_mglev* F = &mg->lev[0];
         const cfo_mglev* C = &mg->lev[1];
@@ -462,7 +586,9 @@ static void mg_vcycle0_f32(struct cfo_state* s, double* z, const double* r)
             for (int j = 0; j < F->n[1]; ++j)
                 for (int i = 0; i < F->n[0]; ++i) {
                     long fi = g0(s, i, j, k);
-                    x[fi] = fmaf(a, (float)mg->lev[1].x[lidx(C, i / 2, j / 2, k / 2)], x[fi]);
+                    long ci = lidx(C, i / 2, j / 2, k / 2);
+                    float xc = f1 ? mg->x1f[ci] : (float)mg->lev[1].x[ci];
+                    x[fi] = fmaf(a, xc, x[fi]);
                 }
     }
     for (int it = 0; it < nu; ++it) {

@@ -63,6 +63,10 @@ CASES = {
     # a coarse level large enough for the z-marching kernels (66^3), then
     # padded row-kernel levels (33^3) and compact ones (17^3)
     "3d_periodic_x_132": _cfg(3, (132, 132, 132), periodic=(1, 0, 0)),
+    # level 1 a marched tile level of 36 x 36 x 28 cells (FP32 with level 0
+    # under CF_FP32_REFINED), FREE top
+    "3d_free_top_72x72x56": _cfg(3, (72, 72, 56),
+                                 walls=[[SOLID, SOLID], [SOLID, SOLID], [SOLID, FREE]]),
 }
 
 
@@ -244,7 +248,8 @@ def test_mg_pcg_fp32_level0_matches_oracle(cf, oracle_mod, name):
 
 @pytest.mark.parametrize("maxit", [1, 2, 5])
 @pytest.mark.parametrize("name", ["3d_ragged_70x36x40", "3d_periodic_xy_21x18x11",
-                                  "3d_nu1_coarse4_20", "3d_odd_45", "3d_periodic_x_132"])
+                                  "3d_nu1_coarse4_20", "3d_odd_45", "3d_periodic_x_132",
+                                  "3d_free_top_72x72x56"])
 def test_mg_pcg_fp32_level0_truncated(cf, oracle_mod, name, maxit):
     from cajitafluids_amd import _lib
     cfg = dict(CASES[name], precision=K.FP32_REFINED, cg_max_iter=maxit)

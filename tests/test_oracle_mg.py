@@ -282,3 +282,31 @@ def test_fp32_mg_is_3d_only(oracle_mod):
     with pytest.raises(ValueError):
         o.mg_vcycle(np.ones(35), f32=True)
     o.close()
+
+
+def test_fp32_level1_vcycle_and_solve(oracle_mod):
+    """72 x 72 x 56 (level 1 has 36 x 36 x 28 > 32768 cells, every axis even):
+    levels 0 and 1 in FP32.  The V-cycle stays within FP32 rounding of the
+    FP64 one, and MG-PCG with it reaches the FP64 MG solution."""
+    cfg = _cfg(3, (72, 72, 56), walls=[[SOLID, SOLID], [SOLID, SOLID], [SOLID, FREE]])
+    o = oracle_mod.Oracle(cfg)
+    N = 72 * 72 * 56
+    r = np.random.default_rng(5).normal(size=N)
+    z64 = o.mg_vcycle(r)
+    z32 = o.mg_vcycle(r, f32=True)
+    d = np.linalg.norm(z32 - z64) / np.linalg.norm(z64)
+    assert 1e-10 < d < 1e-5, d
+    o.close()
+    xs = {}
+    for prec in (K.FP64, K.FP32_REFINED):
+        c = dict(cfg, cg_tol=1e-10, precision=prec)
+        o = oracle_mod.Oracle(c)
+        o.init_random_faces(3)
+        o.build_rhs()
+        o.rhs_remove_mean()
+        rc, st = o.pressure_solve(prec)
+        assert rc == 0 and st["converged"] and st["final_residual"] < 1e-10
+        xs[prec] = o.owned_view(o.lhs(), 0).copy()
+        o.close()
+    x64, x32 = xs[K.FP64], xs[K.FP32_REFINED]
+    assert np.linalg.norm(x32 - x64) <= 1e-8 * np.linalg.norm(x64)
