@@ -19,7 +19,7 @@ STATUS_NAMES = {0: "CF_OK", 1: "CF_EINVAL", 2: "CF_ENOMEM", 3: "CF_ENOCONV",
 
 # kernel ids of cf_kernel_time
 (TK_DIR_SPMV, TK_UPDATE, TK_FINALIZE, TK_DIR_SPMV32, TK_UPDATE32, TK_FINALIZE32,
- TK_ADVECT, TK_RESIDUAL, TK_VCYCLE) = range(9)
+ TK_ADVECT, TK_RESIDUAL, TK_VCYCLE, TK_PCG_COOP) = range(10)
 
 
 class CfConfig(C.Structure):

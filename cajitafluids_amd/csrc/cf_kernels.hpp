@@ -21,6 +21,7 @@ struct PcgState
             // beta and rho to k_mgpcg_rz
     int nonfinite; // ||b|| or ||r|| came out NaN or Inf: the solve stopped
     double alpha_lag; // alpha of an X_SKIP update, applied by the next X_PAIR
+    int stalled;      // a cooperative launch timed out waiting for a workgroup
 };
 
 // Grid-reduction scratch of the streaming kernels: per-block double-double
@@ -121,6 +122,19 @@ inline bool pcg_pairs_x(const Geo& g)
 // Largest grid any reducing kernel of this geometry launches (sizes the
 // partials buffer).
 int reduction_blocks_max(const Geo& g);
+
+// 2-D grids: FP64 Jacobi-PCG iterations in one cooperative launch of one
+// workgroup per strip of rows, until the state's stop test or `max_its`
+// iterations of this launch (x, r and the direction p are read at the start
+// and written back at the end, so launches chain).  The same x, r, direction
+// and state bits as the dir_spmv / update pair.  pcg_coop_fits: the grid
+// qualifies (2-D, at most 2048 cells along x and 256 strips of 2048 cells);
+// xch: coop_xch_bytes(g) bytes of device memory.  A launch that waited too
+// long for another workgroup sets PcgState::stalled.
+bool pcg_coop_fits(const Geo& g);
+size_t coop_xch_bytes(const Geo& g);
+void launch_pcg_solve_coop(const Geo& g, double* x, double* r, double* p, PcgState* st,
+                           void* xch, int max_its, hipStream_t s);
 
 // FP32-inner / FP64-outer defect correction.  The host runs the outer loop
 // from the device-computed norms (one read-back per pass).

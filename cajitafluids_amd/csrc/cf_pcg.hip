@@ -60,6 +60,7 @@ __device__ __forceinline__ void init_state(PcgState* st, double rz, double nn,
     st->res = zero ? 0.0 : (bad ? norm : 1.0);
     st->mg = 0;
     st->nonfinite = bad ? 1 : 0;
+    st->stalled = 0;
 }
 
 // Scalars after an update: iteration count, relative residual, the stop
@@ -226,6 +227,468 @@ __global__ __launch_bounds__(256) void k_pcg_update(
     double hi[2] = {0.0, 0.0}, lo[2] = {0.0, 0.0};
     update_rows<DIM, T>(g, x, r, p, (T)st->alpha, hi, lo);
     grid_reduce<2>(hi, lo, rb, [&](double* sm) { finish_update(st, sm[0], sm[1]); });
+}
+
+// ---- whole solve in one cooperative launch (2-D grids) --------------------------------
+// A 2-D streaming iteration at 128^2-512^2 is two launches whose time goes
+// to start-up, one sweep per lane group and the grid reduction's arrival
+// chain (about 7.5 us each at 128^2), not to bandwidth.  Here one
+// cooperative launch of G workgroups runs all iterations:
+//   - workgroup g owns the rows [g R, g R + R) as a strip; lane t of it owns
+//     strip cells t + 256 m (m < kCoopCpt, x fastest); r, x and q = A p of
+//     those cells stay in registers;
+//   - the direction lives in the strip's LDS window: its rows, one halo row
+//     above and below, and a one-cell ring in x.  Ring cells of periodic
+//     axes are copies of the wrapped cells; the others hold a zero whose
+//     sign is that of the coupling s, so that fma(s, ring, y) adds
+//     s * ring = +0, the term fma(0, 0, y) the streaming kernels add across a
+//     non-periodic boundary (the same bits for every y, -0 included);
+//   - a halo row's direction is computed by the strip itself, by the same
+//     expression, from the neighbour strip's boundary r and old direction,
+//     which that strip publishes with the reduction below;
+//   - each of an iteration's two reductions: a workgroup reduction, the
+//     workgroup's double-double partial published with agent-scope stores
+//     (written through to memory), then an agent-scope store of a tag (the
+//     reduction's sequence number) after the partial's stores completed;
+//     every workgroup polls all tags, reads all partials and sums them in
+//     one fixed order, so every workgroup holds the same alpha, beta and
+//     stop decision.  No fence is issued (an agent-scope fence would write
+//     back and invalidate the XCD's whole L2).
+// A partial or halo slot is rewritten only after its readers have
+// published the next reduction, which they do after reading it, so one
+// buffer per slot suffices.  Every value is the streaming kernels'
+// expression on the same operands, so x, r, the direction and the state
+// come out with the same bits.  A poll that exceeds its bound sets
+// PcgState::stalled and ends the launch (the host reports CF_EDEVICE).
+constexpr int kCoopThreads = 256;
+constexpr int kCoopCpt = 8;                         // cells per lane, at most
+constexpr int kCoopCells = kCoopThreads * kCoopCpt; // cells per strip, at most
+constexpr int kCoopMaxGroups = 256;
+constexpr int kCoopWaves = kCoopThreads / kWave;
+constexpr unsigned kCoopSpin = 1u << 22; // polls (about 1 us each) before a wait counts as stalled
+// unsigned per tag: each strip's tag on its own 64-byte line, so that the
+// polls of all workgroups spread over as many lines as there are strips
+constexpr int kCoopTagStride = 16;
+
+// Strip layout: R rows per strip, G strips; window row pitch e0 = n0 + 2
+// (window index of strip-relative cell (i, jj), halo rows jj = -1 and
+// jj = R: (i+1) + e0 (jj+1)).
+struct CoopGrid
+{
+    int G, R, e0, nwin;
+    int c00; // padded index of owned cell (0, 0)
+    int px;
+};
+
+// Strips of as many whole rows as kCoopCells holds: the fewest workgroups.
+// (Shorter strips, at two cells per lane, took 3% longer at 128^2: the
+// waits for the other strips, not the strip's own work, set the pace.)
+inline CoopGrid coop_grid(const Geo& g)
+{
+    CoopGrid cg;
+    cg.R = kCoopCells / g.n[0];
+    if (cg.R > g.n[1])
+        cg.R = g.n[1];
+    cg.G = (g.n[1] + cg.R - 1) / cg.R;
+    cg.e0 = g.n[0] + 2;
+    cg.nwin = cg.e0 * (cg.R + 2);
+    cg.c00 = (int)g.idx(g.h, g.h, g.own_lo(2));
+    cg.px = (int)g.px;
+    return cg;
+}
+
+// The exchange buffer: tags (B: p.q, C: r.r and r.z) and partials per
+// strip, and each strip's boundary rows (r, then the old direction; top
+// row, then bottom row).
+struct CoopXch
+{
+    unsigned* tag;   // [2][G][kCoopTagStride]
+    double* part;    // [G][8]: B hi, lo; C hi0, lo0, hi1, lo1
+    double* halo;    // [G][2 rows][2 arrays][n0]
+};
+
+// An iteration's reduction in three steps:
+//   coop_publish: the workgroup's double-double partial (wave butterflies,
+//     then thread 0 over the wave sums) to its slot with agent-scope stores;
+//     every lane's stores of this step complete (the halo rows too) before
+//     thread 0 stores the strip's tag;
+//   coop_wait: wave 0 polls every strip's tag; a barrier tells the others;
+//   coop_gather: wave 0 reads all partials and sums them in one fixed order
+//     (the same in every workgroup), while the other waves may load the
+//     halo rows.
+// A lane of wave 0 handles strips lane + 64 u (u < kCoopPerLane); their
+// loads are issued together.  Partial slots: part[strip * 8 + poff + 2 v +
+// {0, 1}].
+constexpr int kCoopPerLane = kCoopMaxGroups / kWave;
+
+template <int NV>
+__device__ __forceinline__ void coop_publish(double (&hi)[NV], double (&lo)[NV], double* part,
+                                             int poff, unsigned* tag, unsigned seq,
+                                             double (*red)[2][kCoopWaves])
+{
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+        dd_norm(hi[v], lo[v]);
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const double oh = __shfl_xor(hi[v], off, kWave);
+            const double ol = __shfl_xor(lo[v], off, kWave);
+            dd_add(hi[v], lo[v], oh, ol);
+        }
+    }
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    if (lane == 0) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            red[v][0][wv] = hi[v];
+            red[v][1][wv] = lo[v];
+        }
+    }
+    // this lane's stores complete before the tag (the waitcnt orders the
+    // hardware; the signal fences keep the compiler from reordering)
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double* mine = part + (size_t)blockIdx.x * 8 + poff;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            double h = red[v][0][0], l = red[v][1][0];
+#pragma unroll
+            for (int w = 1; w < kCoopWaves; ++w)
+                dd_add(h, l, red[v][0][w], red[v][1][w]);
+            __hip_atomic_store(&mine[2 * v], h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&mine[2 * v + 1], l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __hip_atomic_store(&tag[(size_t)blockIdx.x * kCoopTagStride], seq, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__device__ __forceinline__ void coop_wait(const unsigned* tag, unsigned seq, int G, int* stall)
+{
+    if (threadIdx.x < kWave) {
+        unsigned polls = 0;
+        for (;;) {
+            unsigned got[kCoopPerLane];
+#pragma unroll
+            for (int u = 0; u < kCoopPerLane; ++u) {
+                const int q = threadIdx.x + kWave * u;
+                got[u] = q < G ? __hip_atomic_load(&tag[(size_t)q * kCoopTagStride],
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                               : seq;
+            }
+            bool all = true;
+#pragma unroll
+            for (int u = 0; u < kCoopPerLane; ++u)
+                all = all && got[u] >= seq;
+            if (__all(all))
+                break;
+            if (++polls >= kCoopSpin) {
+                if (threadIdx.x == 0)
+                    *stall = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    // the loads that follow stay after the poll (compiler-only barrier)
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __syncthreads();
+}
+
+// in wave 0: the rounded totals in every lane
+template <int NV>
+__device__ __forceinline__ void coop_gather(const double* part, int poff, int G, double (&out)[NV])
+{
+    double ph[kCoopPerLane][NV], pl[kCoopPerLane][NV];
+#pragma unroll
+    for (int u = 0; u < kCoopPerLane; ++u) {
+        const int q = threadIdx.x + kWave * u;
+        const double* pq = part + (size_t)(q < G ? q : 0) * 8 + poff;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            ph[u][v] = q < G ? __hip_atomic_load(&pq[2 * v], __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT)
+                             : 0.0;
+            pl[u][v] = q < G ? __hip_atomic_load(&pq[2 * v + 1], __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT)
+                             : 0.0;
+        }
+    }
+    double h2[NV], l2[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        h2[v] = ph[0][v];
+        l2[v] = pl[0][v];
+#pragma unroll
+        for (int u = 1; u < kCoopPerLane; ++u)
+            dd_add(h2[v], l2[v], ph[u][v], pl[u][v]);
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const double oh = __shfl_xor(h2[v], off, kWave);
+            const double ol = __shfl_xor(l2[v], off, kWave);
+            dd_add(h2[v], l2[v], oh, ol);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+        out[v] = h2[v] + l2[v];
+}
+
+// A lane's cells: slot word = window index (bits 0-15), SOLID wall count
+// (bits 16-18), kCoopOwned (bit 20), and whether the cell is in the strip's
+// top / bottom row (bits 21, 22); a slot past the strip's last cell carries
+// the first cell's window index without kCoopOwned, and its stores and sums
+// are masked.
+constexpr int kCoopOwned = 1 << 20;
+constexpr int kCoopTop = 1 << 21;
+constexpr int kCoopBottom = 1 << 22;
+
+__global__ __launch_bounds__(kCoopThreads) void k_pcg_solve_coop(
+    Geo g, CoopGrid cg, double* __restrict__ x, double* __restrict__ r,
+    double* __restrict__ pg, PcgState* st, CoopXch xc, int max_its)
+{
+    extern __shared__ double lds[];
+    double* pw = lds;             // the direction window
+    double* tab = lds + cg.nwin;  // diag[0..6], then dinv[0..6]
+    double* hr = tab + 14;        // halo rows' r   [2][n0]
+    double* hp = hr + 2 * g.n[0]; // halo rows' old direction [2][n0]
+    __shared__ double red[2][2][kCoopWaves];
+    __shared__ double sh_ab[2]; // alpha, beta
+    __shared__ int sh_stop, sh_stall;
+    // The state is read once, here; every workgroup keeps the same copy in
+    // LDS (only thread 0 writes it), and workgroup 0 stores it back at the
+    // end.  A read of *st after a write in this kernel could be served by
+    // the scalar cache, which vector stores do not update.
+    __shared__ PcgState ls;
+    const int t = threadIdx.x;
+    const int gid = blockIdx.x, G = cg.G;
+    const int n0 = g.n[0], n1 = g.n[1], e0 = cg.e0;
+    const int j0 = gid * cg.R;
+    const int nrows = (j0 + cg.R < n1 ? cg.R : n1 - j0);
+    const double s = offdiag<double>(g);
+    const double ring0 = copysign(0.0, s);
+    if (t == 0) {
+        ls = *st;
+        sh_stall = 0;
+    }
+    for (int e = t; e < cg.nwin; e += kCoopThreads)
+        pw[e] = ring0;
+#pragma unroll
+    for (int q = 0; q < 7; ++q)
+        if (t == q) {
+            tab[q] = g.diag[q];
+            tab[7 + q] = g.dinv[q];
+        }
+    __syncthreads();
+    if (ls.done)
+        return;
+    bool first = ls.iter == 0;
+    double beta = ls.beta;
+
+    // halo rows: window rows 0 (global row j0 - 1) and nrows + 1 (row j0 +
+    // nrows), when they exist; `src` the strip that owns them
+    const bool py = g.periodic[1];
+    const bool has_top = j0 > 0 || py, has_bot = j0 + nrows < n1 || py;
+    const int jtop = j0 > 0 ? j0 - 1 : n1 - 1, jbot = j0 + nrows < n1 ? j0 + nrows : 0;
+    const int gtop = gid > 0 ? gid - 1 : G - 1, gbot = gid + 1 < G ? gid + 1 : 0;
+    // SOLID wall count of a cell of global row j (in 2-D, z adds its own
+    // count for k = 0 as the streaming kernels do)
+    auto cell_cnt = [&](int i, int j) -> int { return row_stencil(g, j, 0).cnt + lane_x(g, i).cnt; };
+    // the two halo rows' r and old direction into LDS from the padded
+    // arrays (start of a launch)
+    if (!first)
+        for (int e = t; e < 2 * n0; e += kCoopThreads) {
+            const int side = e / n0, i = e % n0;
+            if (side == 0 ? has_top : has_bot) {
+                const int64_t c = cg.c00 + i + (int64_t)cg.px * (side == 0 ? jtop : jbot);
+                hr[e] = r[c];
+                hp[e] = pg[c];
+            }
+        }
+    else
+        for (int e = t; e < 2 * n0; e += kCoopThreads) {
+            const int side = e / n0, i = e % n0;
+            if (side == 0 ? has_top : has_bot)
+                hr[e] = r[cg.c00 + i + (int64_t)cg.px * (side == 0 ? jtop : jbot)];
+        }
+
+    int info[kCoopCpt];
+    double rv[kCoopCpt], qv[kCoopCpt], xv[kCoopCpt];
+#pragma unroll
+    for (int m = 0; m < kCoopCpt; ++m) {
+        const int L = t + kCoopThreads * m;
+        const int i = L % n0, jj = L / n0;
+        info[m] = 1 + e0; // window index of strip cell (0, 0)
+        rv[m] = 0.0;
+        qv[m] = 0.0;
+        xv[m] = 0.0;
+        if (jj < nrows) {
+            const int li = (i + 1) + e0 * (jj + 1);
+            info[m] = li | (cell_cnt(i, j0 + jj) << 16) | kCoopOwned |
+                      (jj == 0 ? kCoopTop : 0) | (jj == nrows - 1 ? kCoopBottom : 0);
+            const int64_t c = cg.c00 + i + (int64_t)cg.px * (j0 + jj);
+            rv[m] = r[c];
+            xv[m] = x[c];
+            if (!first)
+                pw[li] = pg[c];
+        }
+    }
+    // padded index of a slot's cell
+    auto padded = [&](int li) -> int64_t {
+        const int jj = li / e0 - 1, i = li - e0 * (jj + 1) - 1;
+        return cg.c00 + i + (int64_t)cg.px * (j0 + jj);
+    };
+    unsigned seq = 0;
+    for (int its = 1;; ++its) {
+        // direction: p = D^-1 r, or fma(beta, p, D^-1 r), for the strip's
+        // cells (staged in qv, free until the stencil) and its halo rows
+        if (first) {
+#pragma unroll
+            for (int m = 0; m < kCoopCpt; ++m)
+                qv[m] = tab[7 + ((info[m] >> 16) & 7)] * rv[m];
+        } else {
+#pragma unroll
+            for (int m = 0; m < kCoopCpt; ++m)
+                qv[m] = fma(beta, pw[info[m] & 0xffff], tab[7 + ((info[m] >> 16) & 7)] * rv[m]);
+        }
+#pragma unroll
+        for (int m = 0; m < kCoopCpt; ++m)
+            if (info[m] & kCoopOwned)
+                pw[info[m] & 0xffff] = qv[m];
+        for (int e = t; e < 2 * n0; e += kCoopThreads) {
+            const int side = e / n0, i = e % n0;
+            if (side == 0 ? has_top : has_bot) {
+                const double z = tab[7 + cell_cnt(i, side == 0 ? jtop : jbot)] * hr[e];
+                pw[(i + 1) + (side == 0 ? 0 : e0 * (nrows + 1))] =
+                    first ? z : fma(beta, hp[e], z);
+            }
+        }
+        __syncthreads();
+        if (g.periodic[0]) {
+            for (int jw = t; jw < nrows + 2; jw += kCoopThreads) {
+                double* row = pw + e0 * jw;
+                row[0] = row[n0];
+                row[n0 + 1] = row[1];
+            }
+            __syncthreads();
+        }
+        // q = A p in the stencil order {0}, -x, +x, -y, +y; p.q
+        double h1[1] = {0.0}, l1[1] = {0.0};
+#pragma unroll
+        for (int m = 0; m < kCoopCpt; ++m) {
+            const int im = info[m];
+            const double* pp = pw + (im & 0xffff);
+            const double pc = pp[0];
+            double y = tab[(im >> 16) & 7] * pc;
+            y = fma(s, pp[-1], y);
+            y = fma(s, pp[1], y);
+            y = fma(s, pp[-e0], y);
+            y = fma(s, pp[e0], y);
+            const bool own = im & kCoopOwned;
+            qv[m] = own ? y : 0.0;
+            dd_acc_prod(h1[0], l1[0], own ? pc : 0.0, qv[m]);
+        }
+        coop_publish<1>(h1, l1, xc.part, 0, xc.tag, ++seq, red);
+        coop_wait(xc.tag, seq, G, &sh_stall);
+        if (sh_stall)
+            break;
+        if (t < kWave) {
+            double pq[1];
+            coop_gather<1>(xc.part, 0, G, pq);
+            if (t == 0) {
+                const double a = ls.rho / pq[0];
+                ls.alpha = a;
+                sh_ab[0] = a;
+            }
+        }
+        __syncthreads();
+        const double alpha = sh_ab[0];
+        // x = fma(alpha, p, x); r = fma(-alpha, q, r); r.r and r.(D^-1 r).
+        // Past the strip's last cell q = r = 0, so rn = 0 and the sums are
+        // unchanged.
+        double hi[2] = {0.0, 0.0}, lo[2] = {0.0, 0.0};
+#pragma unroll
+        for (int m = 0; m < kCoopCpt; ++m) {
+            const int im = info[m];
+            xv[m] = fma(alpha, pw[im & 0xffff], xv[m]);
+            const double rn = fma(-alpha, qv[m], rv[m]);
+            rv[m] = rn;
+            const double z = tab[7 + ((im >> 16) & 7)] * rn;
+            dd_acc_prod(hi[0], lo[0], rn, rn);
+            dd_acc_prod(hi[1], lo[1], rn, z);
+            // boundary rows for the neighbour strips: the new r and this
+            // iteration's direction
+#pragma unroll
+            for (int side = 0; side < 2; ++side)
+                if (im & (side == 0 ? kCoopTop : kCoopBottom)) {
+                    const int li = im & 0xffff;
+                    const int i = li - e0 * (side == 0 ? 1 : nrows) - 1;
+                    double* hb = xc.halo + ((size_t)gid * 2 + side) * 2 * n0;
+                    __hip_atomic_store(&hb[i], rn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(&hb[n0 + i], pw[li], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                }
+        }
+        coop_publish<2>(hi, lo, xc.part, 2, xc.tag + (size_t)G * kCoopTagStride, ++seq, red);
+        coop_wait(xc.tag + (size_t)G * kCoopTagStride, seq, G, &sh_stall);
+        if (sh_stall)
+            break;
+        if (t < kWave) {
+            double rr[2];
+            coop_gather<2>(xc.part, 2, G, rr);
+            if (t == 0) {
+                finish_update(&ls, rr[0], rr[1]);
+                sh_stop = ls.done || its >= max_its;
+                sh_ab[1] = ls.beta;
+            }
+        } else {
+            // the other waves: the neighbours' boundary rows (published
+            // before their tags)
+            for (int e = t - kWave; e < 2 * n0; e += kCoopThreads - kWave) {
+                const int side = e / n0, i = e % n0;
+                if (side == 0 ? has_top : has_bot) {
+                    // the top halo row is the bottom row of the strip above
+                    const double* hb = xc.halo + ((size_t)(side == 0 ? gtop : gbot) * 2 +
+                                                  (side == 0 ? 1 : 0)) * 2 * n0;
+                    hr[e] = __hip_atomic_load(&hb[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    hp[e] = __hip_atomic_load(&hb[n0 + i], __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+        }
+        __syncthreads();
+        if (sh_stop)
+            break;
+        beta = sh_ab[1];
+        first = false;
+    }
+    // x, r and the direction back to memory (a later launch resumes from
+    // them), and the state from workgroup 0
+    if (sh_stall && t == 0)
+        ls.stalled = 1;
+    if (gid == 0 && t == 0)
+        *st = ls;
+#pragma unroll
+    for (int m = 0; m < kCoopCpt; ++m)
+        if (info[m] & kCoopOwned) {
+            const int li = info[m] & 0xffff;
+            const int64_t c = padded(li);
+            x[c] = xv[m];
+            r[c] = rv[m];
+            pg[c] = pw[li];
+        }
 }
 
 // ---- 3-D kernels (cf_tile3.hpp) ------------------------------------------------
@@ -771,6 +1234,52 @@ int reduction_blocks_max(const Geo& g)
 {
     const int t = g.dim == 3 ? tile3_blocks<1>(g) : 0; // the largest tile grid
     return t > kRedBlocks ? t : kRedBlocks;
+}
+
+bool pcg_coop_fits(const Geo& g)
+{
+    if (g.dim != 2 || g.n[0] > kCoopCells)
+        return false;
+    const CoopGrid cg = coop_grid(g);
+    if (cg.G > kCoopMaxGroups)
+        return false;
+    // the window, the coefficient tables and the halo rows fit the LDS;
+    // window indices fit the slot word's 16 bits
+    const int64_t lds = ((int64_t)cg.nwin + 14 + 4 * g.n[0]) * (int64_t)sizeof(double);
+    return cg.nwin < (1 << 16) && lds <= 64 * 1024;
+}
+
+size_t coop_xch_bytes(const Geo& g)
+{
+    const CoopGrid cg = coop_grid(g);
+    return (size_t)cg.G * (8 + 4 * (size_t)g.n[0]) * sizeof(double) +
+           2 * (size_t)cg.G * kCoopTagStride * sizeof(unsigned);
+}
+
+void launch_pcg_solve_coop(const Geo& g, double* x, double* r, double* p, PcgState* st,
+                           void* xch, int max_its, hipStream_t s)
+{
+    if (!pcg_coop_fits(g))
+        throw CfError(CF_EINVAL, "grid does not fit the cooperative PCG");
+    if (max_its < 1)
+        throw CfError(CF_EINVAL, "the cooperative PCG runs at least one iteration per launch");
+    const CoopGrid cg = coop_grid(g);
+    CoopXch xc;
+    xc.part = (double*)xch;
+    xc.halo = xc.part + (size_t)cg.G * 8;
+    xc.tag = (unsigned*)(xc.halo + (size_t)cg.G * 4 * g.n[0]);
+    // tags start at 0 in every launch; sequence numbers count from 1
+    CF_HIP(hipMemsetAsync(xc.tag, 0, 2 * (size_t)cg.G * kCoopTagStride * sizeof(unsigned), s));
+    const size_t lds = ((size_t)cg.nwin + 14 + 4 * (size_t)g.n[0]) * sizeof(double);
+    Geo ga = g;
+    CoopGrid cga = cg;
+    double *xa = x, *ra = r, *pa = p;
+    PcgState* sta = st;
+    CoopXch xca = xc;
+    int ma = max_its;
+    void* args[] = {&ga, &cga, &xa, &ra, &pa, &sta, &xca, &ma};
+    CF_HIP(hipLaunchCooperativeKernel((const void*)k_pcg_solve_coop, dim3(cg.G),
+                                      dim3(kCoopThreads), args, (unsigned)lds, s));
 }
 
 template void launch_pcg_dir_spmv<double>(const Geo&, const double*,

@@ -333,10 +333,37 @@ int VelocityCorrector::runPcg64(const double* b, double* x, double tol,
     RedBuf rb{_red_partials.ptr, _red_counter.ptr};
     launch_pcg_init64(pcgGeo(), b, x, _r.ptr, _st, tol, maxit, rb, _s);
     int cur = 0;
-    const int its =
-        runIterations<double>(x, _r.ptr, _p, cur, _predict64, TK_DIR_SPMV);
+    const int its = pcg_coop_fits(pcgGeo())
+                        ? runCoop(x)
+                        : runIterations<double>(x, _r.ptr, _p, cur, _predict64, TK_DIR_SPMV);
     _predict64 = its;
     return its;
+}
+
+// Iterations of a launch of the cooperative PCG: the first covers the
+// previous solve's count with a margin (at least kCoopItsPerLaunch); the
+// host reads the state after each launch, and a solve that has not stopped
+// goes on in launches of kCoopItsPerLaunch from the stored x, r, direction
+// and state.
+constexpr int kCoopItsPerLaunch = 256;
+
+int VelocityCorrector::runCoop(double* x)
+{
+    if (!_xch.ptr)
+        _xch.alloc((int64_t)(coop_xch_bytes(pcgGeo()) + sizeof(double) - 1) / sizeof(double), _s);
+    PcgState hs;
+    int its = _predict64 + 64 > kCoopItsPerLaunch ? _predict64 + 64 : kCoopItsPerLaunch;
+    for (;; its = kCoopItsPerLaunch) {
+        const int t0 = _timer.begin(_s);
+        launch_pcg_solve_coop(pcgGeo(), x, _r.ptr, _p[0].ptr, _st, _xch.ptr, its, _s);
+        _timer.end(t0, _s, TK_PCG_COOP, 0);
+        readState(hs);
+        _timer.resolve(INT_MAX);
+        if (hs.stalled)
+            throw CfError(CF_EDEVICE, "cooperative PCG: a workgroup did not arrive");
+        if (hs.done)
+            return hs.iter;
+    }
 }
 
 cf_status VelocityCorrector::solve64(const double* b, double* x,

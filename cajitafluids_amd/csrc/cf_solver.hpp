@@ -78,7 +78,8 @@ enum TimedKernel
     TK_ADVECT = 6,
     TK_RESIDUAL = 7,   // FP64 defect-correction residual
     TK_VCYCLE = 8,     // one multigrid V-cycle
-    TK_COUNT = 9
+    TK_PCG_COOP = 9,   // one launch of the cooperative 2-D PCG
+    TK_COUNT = 10
 };
 
 class KernelTimer
@@ -186,6 +187,8 @@ class VelocityCorrector
     int runIterations(T* x, T* r, DevArray<T> (&p)[2], int& cur, int predict,
                       int tk0);
     int runPcg64(const double* b, double* x, double tol, int maxit);
+    // the whole FP64 solve by the cooperative 2-D kernel (pcg_coop_fits)
+    int runCoop(double* x);
     void readState(PcgState& host);
     void readRefine();
     int& passPrediction(int pass);
@@ -199,6 +202,7 @@ class VelocityCorrector
     DevArray<double> _lhs, _rhs, _r, _p[2];
     DevArray<float> _d32, _r32, _p32[2];
     DevArray<double> _red_partials, _scratch;
+    DevArray<double> _xch; // exchange buffer of the cooperative 2-D PCG
     DevArray<unsigned> _red_counter;
     PcgState* _st = nullptr;       // device
     RefineState* _rs = nullptr;    // device

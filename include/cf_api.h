@@ -229,7 +229,9 @@ cf_status cf_init_random_faces(cf_solver* h, uint64_t seed);
 cf_status cf_set_kernel_timing(cf_solver* h, int enable);
 /* kernel: 0 FP64 direction+SpMV, 1 FP64 update, 2 FP64 finalize kernels
  * (unused), 3-5 the same for the FP32 inner solve, 6 advection, 7 FP64
- * defect-correction residual, 8 one multigrid V-cycle (all its kernels) */
+ * defect-correction residual, 8 one multigrid V-cycle (all its kernels),
+ * 9 one launch of the cooperative FP64 PCG that 2-D grids (up to 2048
+ * cells along x and 256 strips of 2048 cells) run in place of 0 and 1 */
 cf_status cf_kernel_time(cf_solver* h, int kernel, double* total_ms,
                          int64_t* launches);
 cf_status cf_reset_kernel_time(cf_solver* h);

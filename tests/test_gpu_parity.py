@@ -112,6 +112,83 @@ def test_pcg_3d_ragged_tiles(cf, oracle_mod, periodic, precision, nz):
     assert rel_l2(x_g, x_o) <= TOL
 
 
+def _cfg2(n0, n1, periodic=(0, 0), walls=None, **kw):
+    c = K.base_config(2, n0)
+    c["num_cell"] = [n0, n1, 1]
+    c["hi"] = [1.0, n1 / n0, 1.0]
+    c.update(clamp_dt=0, dt=0.002, periodic=[periodic[0], periodic[1], 0])
+    if walls is not None:
+        c["wall_type"] = walls
+    c.update(kw)
+    return c
+
+
+# 2-D solves run as one cooperative launch of one workgroup per strip of
+# rows (2048 cells a strip): strip counts 1 to 34, a last strip shorter than
+# the others, periodic axes wrapping inside a strip and across strips, FREE
+# walls, a 1000-cell row, and no preconditioner
+COOP_CASES = {
+    "solid_128": _cfg2(128, 128),
+    "periodic_x_300x200": _cfg2(300, 200, periodic=(1, 0)),
+    "periodic_y_100x90": _cfg2(100, 90, periodic=(0, 1)),
+    "periodic_xy_64x48": _cfg2(64, 48, periodic=(1, 1)),
+    "periodic_y_one_strip_40x30": _cfg2(40, 30, periodic=(0, 1)),
+    "free_walls_77x61": _cfg2(77, 61, walls=[[K.FREE, K.SOLID], [K.SOLID, K.FREE],
+                                             [K.SOLID, K.SOLID]]),
+    "wide_rows_1000x20": _cfg2(1000, 20),
+    "unpreconditioned_96x80": _cfg2(96, 80, periodic=(1, 0), preconditioner=K.PRECOND_NONE),
+}
+
+
+def _coop_solve(cf, oracle_mod, cfg, seed):
+    from cajitafluids_amd import _lib
+    sol, orc = _pair(cf, oracle_mod, cfg)
+    sol.set_kernel_timing(True)
+    sol.init_random_faces(seed)
+    gpu_to_oracle(sol, orc)
+    sol.build_rhs()
+    sol.rhs_remove_mean()
+    orc.build_rhs()
+    orc.rhs_remove_mean()
+    try:
+        st_g = sol.pressure_solve(K.FP64)
+        code = 0
+    except _lib.CfError as e:
+        st_g, code = sol.last_stats, e.code
+    rc, st_o = orc.pressure_solve(K.FP64)
+    assert code == rc
+    assert st_g["iterations"] == st_o["iterations"]
+    # the cooperative kernel ran (and nothing else of the PCG)
+    assert sol.kernel_time(_lib.TK_PCG_COOP)[1] >= 1
+    assert sol.kernel_time(_lib.TK_DIR_SPMV)[1] == 0
+    x_g = sol.owned_view(sol.get_pressure(0), 0)
+    x_o = orc.owned_view(orc.lhs(), 0)
+    assert np.any(x_o != 0.0)
+    assert np.array_equal(x_g, x_o)
+    return st_g
+
+
+@pytest.mark.parametrize("name", list(COOP_CASES))
+def test_pcg_2d_cooperative_strips(cf, oracle_mod, name):
+    """Converged 2-D solves: same iteration count and x bit for bit."""
+    print(f"cooperative pcg: {name}", flush=True)
+    st = _coop_solve(cf, oracle_mod, COOP_CASES[name], 7)
+    assert st["converged"] and st["final_residual"] < 1e-8
+
+
+@pytest.mark.parametrize("name,maxit", [("periodic_x_300x200", 1), ("periodic_x_300x200", 2),
+                                        ("periodic_x_300x200", 5), ("periodic_x_300x200", 300),
+                                        ("periodic_xy_64x48", 1), ("periodic_xy_64x48", 6)])
+def test_pcg_2d_cooperative_truncated(cf, oracle_mod, name, maxit):
+    """Solves stopped by the iteration cap, inside the first launch and (300)
+    in a second one that resumes from the stored state (the first launch of
+    a handle's first solve runs 256 iterations)."""
+    print(f"cooperative pcg truncated: {name} maxit {maxit}", flush=True)
+    cfg = dict(COOP_CASES[name], cg_max_iter=maxit, cg_tol=1e-30)
+    st = _coop_solve(cf, oracle_mod, cfg, 9)
+    assert st["iterations"] == maxit and not st["converged"]
+
+
 @pytest.mark.parametrize("precision", [K.FP64, K.FP32_REFINED])
 @pytest.mark.parametrize("maxit", [1, 2, 5, 6])
 def test_pcg_3d_truncated_runs(cf, oracle_mod, precision, maxit):
