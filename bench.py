@@ -430,7 +430,8 @@ def advect_only():
 def dye_tank_leg(with_cpu):
     """The 2-D 128^2 dye tank, setup + 50 steps (linear fields, Jacobi-PCG
     to 1e-8), as a whole run: on the GPU, and with the oracle on the host
-    cores.  At 128^2 the GPU path is launch- and latency-bound."""
+    cores.  At 128^2 the GPU path is bound by the latency of the PCG's two
+    reductions per iteration (one cooperative launch per solve)."""
     from cajitafluids_amd import configs as K
     from cajitafluids_amd.solver import Solver
     cfg = K.dye_tank_2d(128)
