@@ -273,7 +273,36 @@ struct Tile3Ctx
     int wy[tile3::RY];    // y wall count of each row (wave-uniform)
     bool ywall_m[tile3::RY], ywall_p[tile3::RY];
     bool valid[tile3::RY][CPL];
+    // FP32 marches: the x couplings of the lane's columns, 0 across a wall
+    float cxm32[CPL], cxp32[CPL];
+    // the march's coefficient table in LDS (element type of the march):
+    // diag[k] at k, dinv[k] at 8 + k, k the cell's SOLID wall count
+    const void* tab;
 };
+
+// A cell's diagonal and Jacobi inverse for the wall count ubase + wx
+// (ubase wave-uniform, wx per lane).  FP32 marches read the march's LDS
+// table: one LDS read with a per-lane address (the same address in most
+// lanes), where uniform reads of the table and selects cost these
+// issue-bound kernels scalar registers (spilled) and vector instructions.
+// FP64 marches keep the uniform reads and selects: the table read raised
+// the FP64 update's registers past 128 (four waves per SIMD).
+template <class T, int CPL>
+__device__ __forceinline__ T ctx_diag(const Geo& g, const Tile3Ctx<CPL>& c, int ubase, int wx)
+{
+    if constexpr (std::is_same<T, float>::value)
+        return static_cast<const T*>(c.tab)[ubase + wx];
+    else
+        return diag_pick<T>(g, ubase, wx);
+}
+template <class T, int CPL>
+__device__ __forceinline__ T ctx_dinv(const Geo& g, const Tile3Ctx<CPL>& c, int ubase, int wx)
+{
+    if constexpr (std::is_same<T, float>::value)
+        return static_cast<const T*>(c.tab)[8 + ubase + wx];
+    else
+        return dinv_pick<T>(g, ubase, wx);
+}
 
 template <int CPL>
 __device__ __forceinline__ Tile3Ctx<CPL> tile3_ctx(const Geo& g, const Tile3Grid& t)
@@ -307,6 +336,8 @@ __device__ __forceinline__ Tile3Ctx<CPL> tile3_ctx(const Geo& g, const Tile3Grid
         c.wx[j] = a < 0 ? 0 : wall_count(g, 0, a);
         c.xwall_m[j] = wall_lo(g, 0, gi);
         c.xwall_p[j] = wall_hi(g, 0, gi);
+        c.cxm32[j] = c.xwall_m[j] ? 0.0f : offdiag<float>(g);
+        c.cxp32[j] = c.xwall_p[j] ? 0.0f : offdiag<float>(g);
     }
 #pragma unroll
     for (int q = 0; q < RY; ++q) {
@@ -507,7 +538,9 @@ __device__ __forceinline__ T tile3_value(const Geo& g, const Tile3Ctx<CPL>& c,
     } else {
         T dinv;
         if (s < tile3::RY)
-            dinv = dinv_pick<T>(g, c.wy[s] + m.wz, c.wx[j]); // uniform y, z part
+            dinv = ctx_dinv<T>(g, c, c.wy[s] + m.wz, c.wx[j]); // uniform y, z part
+        else if constexpr (std::is_same<T, float>::value)
+            dinv = ctx_dinv<T>(g, c, m.wz, c.hwxy[j]);
         else
             dinv = dinv_sel<T>(g, m.wz, c.hwxy[j]);
         const T z = dinv * P.a[s][j];
@@ -604,9 +637,14 @@ __device__ __forceinline__ T tile3_apply(const Geo& g, const Tile3Ctx<CPL>& c,
                                          int j, const T (&pc)[CPL], T pm, T pp)
 {
     const T s = offdiag<T>(g);
-    T y = diag_pick<T>(g, c.wy[q] + m.wz, c.wx[j]) * pc[j];
-    y = fma_t<T>(c.xwall_m[j] ? (T)0 : s, nb.xm[j], y);
-    y = fma_t<T>(c.xwall_p[j] ? (T)0 : s, nb.xp[j], y);
+    T y = ctx_diag<T>(g, c, c.wy[q] + m.wz, c.wx[j]) * pc[j];
+    if constexpr (std::is_same<T, float>::value) {
+        y = fmaf(c.cxm32[j], nb.xm[j], y);
+        y = fmaf(c.cxp32[j], nb.xp[j], y);
+    } else {
+        y = fma_t<T>(c.xwall_m[j] ? (T)0 : s, nb.xm[j], y);
+        y = fma_t<T>(c.xwall_p[j] ? (T)0 : s, nb.xp[j], y);
+    }
     y = fma_t<T>(c.ywall_m[q] ? (T)0 : s, nb.ym[j], y);
     y = fma_t<T>(c.ywall_p[q] ? (T)0 : s, nb.yp[j], y);
     y = fma_t<T>(m.zwall_m ? (T)0 : s, pm, y);
@@ -647,9 +685,21 @@ __device__ __forceinline__ void tile3_march(
 {
     using namespace tile3;
     using E = std::decay_t<Epi>;
-    const Tile3Ctx<CPL> c = tile3_ctx<CPL>(g, tg);
+    Tile3Ctx<CPL> c = tile3_ctx<CPL>(g, tg);
     if (!c.active)
-        return;
+        return; // the whole block (active depends on the block only)
+    c.tab = nullptr;
+    if constexpr (std::is_same<T, float>::value) {
+        __shared__ T tab[16];
+#pragma unroll
+        for (int q = 0; q < 7; ++q)
+            if (threadIdx.x == q) {
+                tab[q] = diag_of<T>(g, q);
+                tab[8 + q] = dinv_of<T>(g, q);
+            }
+        __syncthreads();
+        c.tab = tab;
+    }
     const bool need_b = DIR == 2 || !first;
     // MODE 1 reads vb on every path (first iteration included, where its
     // values are ignored); a caller without an old direction gets va's (a
