@@ -859,7 +859,7 @@ struct UpdEpiT
         for (int j = 0; j < CPL; ++j) {
             if (!c.valid[q][j])
                 continue;
-            const T z = ctx_dinv<T>(g, c, c.wy[q] + m.wz, c.wx[j]) * ro[j];
+            const T z = ctx_dinv<T, 0>(g, c, c.wy[q] + m.wz, c.wx[j]) * ro[j];
             dd_acc_prod<exact_products<T>()>(hi[0], lo[0], (double)ro[j], (double)ro[j]);
             dd_acc_prod<exact_products<T>()>(hi[1], lo[1], (double)ro[j], (double)z);
         }

@@ -273,8 +273,6 @@ struct Tile3Ctx
     int wy[tile3::RY];    // y wall count of each row (wave-uniform)
     bool ywall_m[tile3::RY], ywall_p[tile3::RY];
     bool valid[tile3::RY][CPL];
-    // FP32 marches: the x couplings of the lane's columns, 0 across a wall
-    float cxm32[CPL], cxp32[CPL];
     // the march's coefficient table in LDS (element type of the march):
     // diag[k] at k, dinv[k] at 8 + k, k the cell's SOLID wall count
     const void* tab;
@@ -287,18 +285,23 @@ struct Tile3Ctx
 // issue-bound kernels scalar registers (spilled) and vector instructions.
 // FP64 marches keep the uniform reads and selects: the table read raised
 // the FP64 update's registers past 128 (four waves per SIMD).
-template <class T, int CPL>
+template <class T, int DIR>
+constexpr bool tile3_tab()
+{
+    return std::is_same<T, float>::value || DIR == 1;
+}
+template <class T, int DIR, int CPL>
 __device__ __forceinline__ T ctx_diag(const Geo& g, const Tile3Ctx<CPL>& c, int ubase, int wx)
 {
-    if constexpr (std::is_same<T, float>::value)
+    if constexpr (tile3_tab<T, DIR>())
         return static_cast<const T*>(c.tab)[ubase + wx];
     else
         return diag_pick<T>(g, ubase, wx);
 }
-template <class T, int CPL>
+template <class T, int DIR, int CPL>
 __device__ __forceinline__ T ctx_dinv(const Geo& g, const Tile3Ctx<CPL>& c, int ubase, int wx)
 {
-    if constexpr (std::is_same<T, float>::value)
+    if constexpr (tile3_tab<T, DIR>())
         return static_cast<const T*>(c.tab)[8 + ubase + wx];
     else
         return dinv_pick<T>(g, ubase, wx);
@@ -336,8 +339,6 @@ __device__ __forceinline__ Tile3Ctx<CPL> tile3_ctx(const Geo& g, const Tile3Grid
         c.wx[j] = a < 0 ? 0 : wall_count(g, 0, a);
         c.xwall_m[j] = wall_lo(g, 0, gi);
         c.xwall_p[j] = wall_hi(g, 0, gi);
-        c.cxm32[j] = c.xwall_m[j] ? 0.0f : offdiag<float>(g);
-        c.cxp32[j] = c.xwall_p[j] ? 0.0f : offdiag<float>(g);
     }
 #pragma unroll
     for (int q = 0; q < RY; ++q) {
@@ -538,11 +539,9 @@ __device__ __forceinline__ T tile3_value(const Geo& g, const Tile3Ctx<CPL>& c,
     } else {
         T dinv;
         if (s < tile3::RY)
-            dinv = ctx_dinv<T>(g, c, c.wy[s] + m.wz, c.wx[j]); // uniform y, z part
-        else if constexpr (std::is_same<T, float>::value)
-            dinv = ctx_dinv<T>(g, c, m.wz, c.hwxy[j]);
+            dinv = ctx_dinv<T, MODE>(g, c, c.wy[s] + m.wz, c.wx[j]); // uniform y, z part
         else
-            dinv = dinv_sel<T>(g, m.wz, c.hwxy[j]);
+            dinv = ctx_dinv<T, MODE>(g, c, m.wz, c.hwxy[j]);
         const T z = dinv * P.a[s][j];
         v = first ? z : fma_t<T>(beta, P.b[s][j], z);
     }
@@ -631,20 +630,15 @@ __device__ __forceinline__ RowNb<T, CPL> tile3_neighbours(const Tile3Ctx<CPL>& c
     return nb;
 }
 
-template <class T, int CPL>
+template <class T, int DIR, int CPL>
 __device__ __forceinline__ T tile3_apply(const Geo& g, const Tile3Ctx<CPL>& c,
                                          const Plane3Meta& m, const RowNb<T, CPL>& nb, int q,
                                          int j, const T (&pc)[CPL], T pm, T pp)
 {
     const T s = offdiag<T>(g);
-    T y = ctx_diag<T>(g, c, c.wy[q] + m.wz, c.wx[j]) * pc[j];
-    if constexpr (std::is_same<T, float>::value) {
-        y = fmaf(c.cxm32[j], nb.xm[j], y);
-        y = fmaf(c.cxp32[j], nb.xp[j], y);
-    } else {
-        y = fma_t<T>(c.xwall_m[j] ? (T)0 : s, nb.xm[j], y);
-        y = fma_t<T>(c.xwall_p[j] ? (T)0 : s, nb.xp[j], y);
-    }
+    T y = ctx_diag<T, DIR>(g, c, c.wy[q] + m.wz, c.wx[j]) * pc[j];
+    y = fma_t<T>(c.xwall_m[j] ? (T)0 : s, nb.xm[j], y);
+    y = fma_t<T>(c.xwall_p[j] ? (T)0 : s, nb.xp[j], y);
     y = fma_t<T>(c.ywall_m[q] ? (T)0 : s, nb.ym[j], y);
     y = fma_t<T>(c.ywall_p[q] ? (T)0 : s, nb.yp[j], y);
     y = fma_t<T>(m.zwall_m ? (T)0 : s, pm, y);
@@ -689,7 +683,7 @@ __device__ __forceinline__ void tile3_march(
     if (!c.active)
         return; // the whole block (active depends on the block only)
     c.tab = nullptr;
-    if constexpr (std::is_same<T, float>::value) {
+    if constexpr (tile3_tab<T, DIR>()) {
         __shared__ T tab[16];
 #pragma unroll
         for (int q = 0; q < 7; ++q)
@@ -755,7 +749,7 @@ __device__ __forceinline__ void tile3_march(
                 T y[CPL];
 #pragma unroll
                 for (int j = 0; j < CPL; ++j)
-                    y[j] = tile3_apply<T, CPL>(g, c, mc, nb, q, j, pc[q],
+                    y[j] = tile3_apply<T, DIR, CPL>(g, c, mc, nb, q, j, pc[q],
                                                REV ? pp[q][j] : pm[q][j],
                                                REV ? pm[q][j] : pp[q][j]);
                 epi.template cells<S>(c, mc, q, pc[q], y);
@@ -797,7 +791,7 @@ __device__ __forceinline__ void tile3_march(
                     T y[CPL];
 #pragma unroll
                     for (int j = 0; j < CPL; ++j)
-                        y[j] = tile3_apply<T, CPL>(g, c, mc, nb, q, j, pc[q],
+                        y[j] = tile3_apply<T, DIR, CPL>(g, c, mc, nb, q, j, pc[q],
                                                    REV ? pp[q][j] : pm[q][j],
                                                    REV ? pm[q][j] : pp[q][j]);
                     epi.template cells<0>(c, mc, q, pc[q], y);
