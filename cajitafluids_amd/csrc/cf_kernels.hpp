@@ -130,10 +130,12 @@ int reduction_blocks_max(const Geo& g);
 // and state bits as the dir_spmv / update pair.  pcg_coop_fits: the grid
 // qualifies (2-D, at most 2048 cells along x and 256 strips of 2048 cells);
 // xch: coop_xch_bytes(g) bytes of device memory.  A launch that waited too
-// long for another workgroup sets PcgState::stalled.
+// long for another workgroup sets PcgState::stalled.  Returns false when the
+// runtime refuses the cooperative launch (the workgroups cannot all be
+// resident, or the device has no cooperative launch); nothing ran then.
 bool pcg_coop_fits(const Geo& g);
 size_t coop_xch_bytes(const Geo& g);
-void launch_pcg_solve_coop(const Geo& g, double* x, double* r, double* p, PcgState* st,
+bool launch_pcg_solve_coop(const Geo& g, double* x, double* r, double* p, PcgState* st,
                            void* xch, int max_its, hipStream_t s);
 
 // FP32-inner / FP64-outer defect correction.  The host runs the outer loop

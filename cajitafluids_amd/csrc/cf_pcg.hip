@@ -1256,7 +1256,7 @@ size_t coop_xch_bytes(const Geo& g)
            2 * (size_t)cg.G * kCoopTagStride * sizeof(unsigned);
 }
 
-void launch_pcg_solve_coop(const Geo& g, double* x, double* r, double* p, PcgState* st,
+bool launch_pcg_solve_coop(const Geo& g, double* x, double* r, double* p, PcgState* st,
                            void* xch, int max_its, hipStream_t s)
 {
     if (!pcg_coop_fits(g))
@@ -1278,8 +1278,14 @@ void launch_pcg_solve_coop(const Geo& g, double* x, double* r, double* p, PcgSta
     CoopXch xca = xc;
     int ma = max_its;
     void* args[] = {&ga, &cga, &xa, &ra, &pa, &sta, &xca, &ma};
-    CF_HIP(hipLaunchCooperativeKernel((const void*)k_pcg_solve_coop, dim3(cg.G),
-                                      dim3(kCoopThreads), args, (unsigned)lds, s));
+    const hipError_t e = hipLaunchCooperativeKernel((const void*)k_pcg_solve_coop, dim3(cg.G),
+                                                    dim3(kCoopThreads), args, (unsigned)lds, s);
+    if (e == hipErrorCooperativeLaunchTooLarge || e == hipErrorNotSupported) {
+        (void)hipGetLastError(); // refused before anything ran
+        return false;
+    }
+    CF_HIP(e);
+    return true;
 }
 
 template void launch_pcg_dir_spmv<double>(const Geo&, const double*,

@@ -333,9 +333,11 @@ int VelocityCorrector::runPcg64(const double* b, double* x, double tol,
     RedBuf rb{_red_partials.ptr, _red_counter.ptr};
     launch_pcg_init64(pcgGeo(), b, x, _r.ptr, _st, tol, maxit, rb, _s);
     int cur = 0;
-    const int its = pcg_coop_fits(pcgGeo())
-                        ? runCoop(x)
-                        : runIterations<double>(x, _r.ptr, _p, cur, _predict64, TK_DIR_SPMV);
+    int its = !_coop_off && pcg_coop_fits(pcgGeo()) ? runCoop(x) : -1;
+    // the streaming kernels, or where a cooperative launch was refused, on
+    // from the state it left (p[0] holds the current direction)
+    if (its < 0)
+        its = runIterations<double>(x, _r.ptr, _p, cur, _predict64, TK_DIR_SPMV);
     _predict64 = its;
     return its;
 }
@@ -344,7 +346,8 @@ int VelocityCorrector::runPcg64(const double* b, double* x, double tol,
 // previous solve's count with a margin (at least kCoopItsPerLaunch); the
 // host reads the state after each launch, and a solve that has not stopped
 // goes on in launches of kCoopItsPerLaunch from the stored x, r, direction
-// and state.
+// and state.  Returns -1 (and turns the path off for the handle) if the
+// runtime refuses a cooperative launch.
 constexpr int kCoopItsPerLaunch = 256;
 
 int VelocityCorrector::runCoop(double* x)
@@ -355,8 +358,14 @@ int VelocityCorrector::runCoop(double* x)
     int its = _predict64 + 64 > kCoopItsPerLaunch ? _predict64 + 64 : kCoopItsPerLaunch;
     for (;; its = kCoopItsPerLaunch) {
         const int t0 = _timer.begin(_s);
-        launch_pcg_solve_coop(pcgGeo(), x, _r.ptr, _p[0].ptr, _st, _xch.ptr, its, _s);
+        const bool ran = launch_pcg_solve_coop(pcgGeo(), x, _r.ptr, _p[0].ptr, _st, _xch.ptr,
+                                               its, _s);
         _timer.end(t0, _s, TK_PCG_COOP, 0);
+        if (!ran) {
+            _timer.resolve(INT_MAX);
+            _coop_off = true;
+            return -1;
+        }
         readState(hs);
         _timer.resolve(INT_MAX);
         if (hs.stalled)

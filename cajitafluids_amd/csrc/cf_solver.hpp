@@ -203,6 +203,7 @@ class VelocityCorrector
     DevArray<float> _d32, _r32, _p32[2];
     DevArray<double> _red_partials, _scratch;
     DevArray<double> _xch; // exchange buffer of the cooperative 2-D PCG
+    bool _coop_off = false; // the runtime refused a cooperative launch
     DevArray<unsigned> _red_counter;
     PcgState* _st = nullptr;       // device
     RefineState* _rs = nullptr;    // device
