@@ -303,25 +303,28 @@ inline CoopGrid coop_grid(const Geo& g)
 struct CoopXch
 {
     unsigned* tag;   // [2][G][kCoopTagStride]
-    double* part;    // [G][8]: B hi, lo; C hi0, lo0, hi1, lo1
+    double* part;    // [G][kCoopWaves][8]: B hi, lo; C hi0, lo0, hi1, lo1
     double* halo;    // [G][2 rows][2 arrays][n0]
 };
 
 // An iteration's reduction in three steps:
-//   coop_publish: the workgroup's double-double partial (wave butterflies,
-//     then thread 0 over the wave sums) to its slot with agent-scope stores;
-//     every lane's stores of this step complete (the halo rows too) before
-//     thread 0 stores the strip's tag;
+//   coop_publish: each wave's double-double partial (wave butterfly) to its
+//     slot with agent-scope stores (WP), or the workgroup's (thread 0 over
+//     the wave sums, to slot strip); every lane's stores
+//     of this step complete (the halo rows too) before thread 0 stores the
+//     strip's tag;
 //   coop_wait: wave 0 polls every strip's tag; a barrier tells the others;
-//   coop_gather: wave 0 reads all partials and sums them in one fixed order
-//     (the same in every workgroup), while the other waves may load the
-//     halo rows.
-// A lane of wave 0 handles strips lane + 64 u (u < kCoopPerLane); their
-// loads are issued together.  Partial slots: part[strip * 8 + poff + 2 v +
-// {0, 1}].
+//   coop_gather: wave 0 reads all wave partials and sums them in one fixed
+//     order (the same in every workgroup), while the other waves may load
+//     the halo rows.
+// A lane of wave 0 polls strips lane + 64 u (u < kCoopPerLane) and reads
+// partials lane + 64 u (u < PPL, the kernel's bucket for the partial
+// count); the loads of a step are issued together.  Partial slots, one
+// 64-byte line each: part[slot 8 + poff + 2 v + {0, 1}], slot = strip
+// kCoopWaves + wave (WP) or strip.
 constexpr int kCoopPerLane = kCoopMaxGroups / kWave;
 
-template <int NV>
+template <int NV, bool WP>
 __device__ __forceinline__ void coop_publish(double (&hi)[NV], double (&lo)[NV], double* part,
                                              int poff, unsigned* tag, unsigned seq,
                                              double (*red)[2][kCoopWaves])
@@ -341,10 +344,21 @@ __device__ __forceinline__ void coop_publish(double (&hi)[NV], double (&lo)[NV],
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = threadIdx.x / kWave;
     if (lane == 0) {
+        if constexpr (WP) {
+            double* mine = part + ((size_t)blockIdx.x * kCoopWaves + wv) * 8 + poff;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            red[v][0][wv] = hi[v];
-            red[v][1][wv] = lo[v];
+            for (int v = 0; v < NV; ++v) {
+                __hip_atomic_store(&mine[2 * v], hi[v], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&mine[2 * v + 1], lo[v], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                red[v][0][wv] = hi[v];
+                red[v][1][wv] = lo[v];
+            }
         }
     }
     // this lane's stores complete before the tag (the waitcnt orders the
@@ -354,19 +368,23 @@ __device__ __forceinline__ void coop_publish(double (&hi)[NV], double (&lo)[NV],
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __syncthreads();
     if (threadIdx.x == 0) {
-        double* mine = part + (size_t)blockIdx.x * 8 + poff;
+        if constexpr (!WP) {
+            // the workgroup's partial: slot blockIdx.x
+            double* mine = part + (size_t)blockIdx.x * 8 + poff;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            double h = red[v][0][0], l = red[v][1][0];
+            for (int v = 0; v < NV; ++v) {
+                double h = red[v][0][0], l = red[v][1][0];
 #pragma unroll
-            for (int w = 1; w < kCoopWaves; ++w)
-                dd_add(h, l, red[v][0][w], red[v][1][w]);
-            __hip_atomic_store(&mine[2 * v], h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&mine[2 * v + 1], l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                for (int w = 1; w < kCoopWaves; ++w)
+                    dd_add(h, l, red[v][0][w], red[v][1][w]);
+                __hip_atomic_store(&mine[2 * v], h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&mine[2 * v + 1], l, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            __builtin_amdgcn_s_waitcnt(0);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
         }
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        __builtin_amdgcn_s_waitcnt(0);
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
         __hip_atomic_store(&tag[(size_t)blockIdx.x * kCoopTagStride], seq, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -405,22 +423,25 @@ __device__ __forceinline__ void coop_wait(const unsigned* tag, unsigned seq, int
 }
 
 // in wave 0: the rounded totals in every lane
-template <int NV>
+template <int NV, int PPL, bool WP>
 __device__ __forceinline__ void coop_gather(const double* part, int poff, int G, double (&out)[NV])
 {
-    double ph[kCoopPerLane][NV], pl[kCoopPerLane][NV];
+    // wave partials (slot strip * kCoopWaves + wave), or one partial per
+    // strip (slot strip)
+    const int nparts = WP ? G * kCoopWaves : G;
+    double ph[PPL][NV], pl[PPL][NV];
 #pragma unroll
-    for (int u = 0; u < kCoopPerLane; ++u) {
+    for (int u = 0; u < PPL; ++u) {
         const int q = threadIdx.x + kWave * u;
-        const double* pq = part + (size_t)(q < G ? q : 0) * 8 + poff;
+        const double* pq = part + (size_t)(q < nparts ? q : 0) * 8 + poff;
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            ph[u][v] = q < G ? __hip_atomic_load(&pq[2 * v], __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT)
-                             : 0.0;
-            pl[u][v] = q < G ? __hip_atomic_load(&pq[2 * v + 1], __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT)
-                             : 0.0;
+            ph[u][v] = q < nparts ? __hip_atomic_load(&pq[2 * v], __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT)
+                                  : 0.0;
+            pl[u][v] = q < nparts ? __hip_atomic_load(&pq[2 * v + 1], __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT)
+                                  : 0.0;
         }
     }
     double h2[NV], l2[NV];
@@ -429,7 +450,7 @@ __device__ __forceinline__ void coop_gather(const double* part, int poff, int G,
         h2[v] = ph[0][v];
         l2[v] = pl[0][v];
 #pragma unroll
-        for (int u = 1; u < kCoopPerLane; ++u)
+        for (int u = 1; u < PPL; ++u)
             dd_add(h2[v], l2[v], ph[u][v], pl[u][v]);
     }
 #pragma unroll
@@ -455,6 +476,7 @@ constexpr int kCoopOwned = 1 << 20;
 constexpr int kCoopTop = 1 << 21;
 constexpr int kCoopBottom = 1 << 22;
 
+template <int PPL, bool WP>
 __global__ __launch_bounds__(kCoopThreads) void k_pcg_solve_coop(
     Geo g, CoopGrid cg, double* __restrict__ x, double* __restrict__ r,
     double* __restrict__ pg, PcgState* st, CoopXch xc, int max_its)
@@ -600,13 +622,13 @@ __global__ __launch_bounds__(kCoopThreads) void k_pcg_solve_coop(
             qv[m] = own ? y : 0.0;
             dd_acc_prod(h1[0], l1[0], own ? pc : 0.0, qv[m]);
         }
-        coop_publish<1>(h1, l1, xc.part, 0, xc.tag, ++seq, red);
+        coop_publish<1, WP>(h1, l1, xc.part, 0, xc.tag, ++seq, red);
         coop_wait(xc.tag, seq, G, &sh_stall);
         if (sh_stall)
             break;
         if (t < kWave) {
             double pq[1];
-            coop_gather<1>(xc.part, 0, G, pq);
+            coop_gather<1, PPL, WP>(xc.part, 0, G, pq);
             if (t == 0) {
                 const double a = ls.rho / pq[0];
                 ls.alpha = a;
@@ -641,13 +663,13 @@ __global__ __launch_bounds__(kCoopThreads) void k_pcg_solve_coop(
                                        __HIP_MEMORY_SCOPE_AGENT);
                 }
         }
-        coop_publish<2>(hi, lo, xc.part, 2, xc.tag + (size_t)G * kCoopTagStride, ++seq, red);
+        coop_publish<2, WP>(hi, lo, xc.part, 2, xc.tag + (size_t)G * kCoopTagStride, ++seq, red);
         coop_wait(xc.tag + (size_t)G * kCoopTagStride, seq, G, &sh_stall);
         if (sh_stall)
             break;
         if (t < kWave) {
             double rr[2];
-            coop_gather<2>(xc.part, 2, G, rr);
+            coop_gather<2, PPL, WP>(xc.part, 2, G, rr);
             if (t == 0) {
                 finish_update(&ls, rr[0], rr[1]);
                 sh_stop = ls.done || its >= max_its;
@@ -1252,7 +1274,7 @@ bool pcg_coop_fits(const Geo& g)
 size_t coop_xch_bytes(const Geo& g)
 {
     const CoopGrid cg = coop_grid(g);
-    return (size_t)cg.G * (8 + 4 * (size_t)g.n[0]) * sizeof(double) +
+    return (size_t)cg.G * (8 * kCoopWaves + 4 * (size_t)g.n[0]) * sizeof(double) +
            2 * (size_t)cg.G * kCoopTagStride * sizeof(unsigned);
 }
 
@@ -1266,7 +1288,7 @@ bool launch_pcg_solve_coop(const Geo& g, double* x, double* r, double* p, PcgSta
     const CoopGrid cg = coop_grid(g);
     CoopXch xc;
     xc.part = (double*)xch;
-    xc.halo = xc.part + (size_t)cg.G * 8;
+    xc.halo = xc.part + (size_t)cg.G * 8 * kCoopWaves;
     xc.tag = (unsigned*)(xc.halo + (size_t)cg.G * 4 * g.n[0]);
     // tags start at 0 in every launch; sequence numbers count from 1
     CF_HIP(hipMemsetAsync(xc.tag, 0, 2 * (size_t)cg.G * kCoopTagStride * sizeof(unsigned), s));
@@ -1278,8 +1300,19 @@ bool launch_pcg_solve_coop(const Geo& g, double* x, double* r, double* p, PcgSta
     CoopXch xca = xc;
     int ma = max_its;
     void* args[] = {&ga, &cga, &xa, &ra, &pa, &sta, &xca, &ma};
-    const hipError_t e = hipLaunchCooperativeKernel((const void*)k_pcg_solve_coop, dim3(cg.G),
-                                                    dim3(kCoopThreads), args, (unsigned)lds, s);
+    // Each wave publishes its own partial while the gather reads at most 4
+    // per lane (up to 64 strips: one store-acknowledgement wait per
+    // reduction); beyond that each workgroup publishes one partial, summed
+    // over its waves (a second wait, but 4 times fewer reads).  Measured per
+    // 2-D iteration: 8.6 us at 128^2 (wave partials) against 9.1, 12.8 us at
+    // 512^2 (workgroup partials) against 13.9.
+    const int nwp = cg.G * kCoopWaves;
+    const void* fn = nwp <= kWave       ? (const void*)k_pcg_solve_coop<1, true>
+                     : nwp <= 4 * kWave ? (const void*)k_pcg_solve_coop<4, true>
+                                        : (const void*)k_pcg_solve_coop<4, false>;
+    static_assert(kCoopMaxGroups <= 4 * kWave, "gather bucket");
+    const hipError_t e = hipLaunchCooperativeKernel(fn, dim3(cg.G), dim3(kCoopThreads), args,
+                                                    (unsigned)lds, s);
     if (e == hipErrorCooperativeLaunchTooLarge || e == hipErrorNotSupported) {
         (void)hipGetLastError(); // refused before anything ran
         return false;
