@@ -869,15 +869,18 @@ void launch_mg0_smooth(const Geo& g, const MgTab& wd0, int src, bool rz, const d
         mg0_smooth_dim<3>(g, wd0, src, rz, xi, b, pm, alpha, xc, xo, st, rb, s);
 }
 
-// the tile grid of the tiled multigrid kernels: an even z chunk
-// (shorter on small levels, whose few tiles would otherwise leave most CUs
-// idle while each block marches its planes one after the other)
+// the tile grid of the tiled multigrid kernels: an even z chunk, shortened
+// on small levels until there are 512 blocks (whose few tiles would
+// otherwise leave most CUs idle while each block marches its planes one
+// after the other; each halving adds two halo planes per chunk, and at 256
+// blocks the V-cycle took longer: 693 against 681 us at 256^3, 1024 blocks
+// 682 against 678 at 512)
 template <int CPL = 1>
 static Tile3Grid mg_tile_grid(const Geo& g)
 {
     Tile3Grid t = tile3_grid<CPL>(g);
     t.kz += t.kz & 1;
-    while (t.kz > 2 && t.tiles_x * t.tiles_y * ((g.n[2] + t.kz - 1) / t.kz) < 1024)
+    while (t.kz > 2 && t.tiles_x * t.tiles_y * ((g.n[2] + t.kz - 1) / t.kz) < 512)
         t.kz = (t.kz / 2 + 1) & ~1;
     t.nchunk = (g.n[2] + t.kz - 1) / t.kz;
     t.work = t.tiles_x * t.tiles_y * t.nchunk;
