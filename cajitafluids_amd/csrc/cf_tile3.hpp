@@ -158,17 +158,6 @@ __device__ __forceinline__ int wrap_or_ghost(const Geo& g, int d, int gi)
     return -1;
 }
 
-// Jacobi diagonal inverse for a wave-uniform z wall count wz and a per-lane
-// x + y wall count wxy in [0, 4]: five uniform table reads and four selects
-// (a table indexed by the per-lane count would be a vector memory read).
-template <class T>
-__device__ __forceinline__ T dinv_sel(const Geo& g, int wz, int wxy)
-{
-    const T a = dinv_of<T>(g, wz), b = dinv_of<T>(g, wz + 1), c = dinv_of<T>(g, wz + 2),
-            d = dinv_of<T>(g, wz + 3), e = dinv_of<T>(g, wz + 4);
-    return wxy == 0 ? a : wxy == 1 ? b : wxy == 2 ? c : wxy == 3 ? d : e;
-}
-
 // Whether the -/+ coupling of an owned cell with global index gi along axis
 // d is cut by a wall (BoundaryConditions.hpp:57-97: gi <= min, gi > max - 1).
 __device__ __forceinline__ bool wall_lo(const Geo& g, int d, int gi)
