@@ -237,7 +237,7 @@ struct MgSweepEpi
 // tile shape (two cells per lane for an FP32 level, with its periodic-x
 // variant).
 template <class T, int MODE, bool RZ, int CPL = 1, bool PX = false, class TP = double>
-__global__ __launch_bounds__(256) void k_mg_smooth3(Geo g, Geo gw, Tile3Grid tg,
+__global__ __launch_bounds__(kTileBlock) void k_mg_smooth3(Geo g, Geo gw, Tile3Grid tg,
                                                     const T* __restrict__ xi,
                                                     const MgRhsT<T, RZ>* __restrict__ b,
                                                     Tile3ParentT<TP> par, T* __restrict__ xo,
@@ -256,7 +256,7 @@ __global__ __launch_bounds__(256) void k_mg_smooth3(Geo g, Geo gw, Tile3Grid tg,
     }
     if constexpr (RZ) {
         dd_norm(epi.hi[0], epi.lo[0]);
-        grid_reduce<1>(epi.hi, epi.lo, rb, [&](double* sm) {
+        grid_reduce<1, kTileBlock>(epi.hi, epi.lo, rb, [&](double* sm) {
             if (st->iter > 0)
                 st->beta = sm[0] / st->rho;
             st->rho = sm[0];
@@ -369,7 +369,7 @@ struct MgRestrictEpi
 };
 
 template <class T, int CPL = 1, bool PX = false, class TO = double>
-__global__ __launch_bounds__(256) void k_mg_resrestrict3(Geo g, Tile3Grid tg, MgParentMap pm,
+__global__ __launch_bounds__(kTileBlock) void k_mg_resrestrict3(Geo g, Tile3Grid tg, MgParentMap pm,
                                                          const T* __restrict__ x,
                                                          const T* __restrict__ b,
                                                          TO* __restrict__ bc,
@@ -870,7 +870,7 @@ void launch_mg0_smooth(const Geo& g, const MgTab& wd0, int src, bool rz, const d
 }
 
 // the tile grid of the tiled multigrid kernels: an even z chunk, shortened
-// on small levels until there are 512 blocks (whose few tiles would
+// on small levels until there are 512 blocks of 256 lanes (whose few tiles would
 // otherwise leave most CUs idle while each block marches its planes one
 // after the other; each halving adds two halo planes per chunk, and at 256
 // blocks the V-cycle took longer: 693 against 681 us at 256^3, 1024 blocks
@@ -880,7 +880,8 @@ static Tile3Grid mg_tile_grid(const Geo& g)
 {
     Tile3Grid t = tile3_grid<CPL>(g);
     t.kz += t.kz & 1;
-    while (t.kz > 2 && t.tiles_x * t.tiles_y * ((g.n[2] + t.kz - 1) / t.kz) < 512)
+    while (t.kz > 2 && t.tiles_x * t.tiles_y * ((g.n[2] + t.kz - 1) / t.kz) * kTileBlock <
+                           512 * 256)
         t.kz = (t.kz / 2 + 1) & ~1;
     t.nchunk = (g.n[2] + t.kz - 1) / t.kz;
     t.work = t.tiles_x * t.tiles_y * t.nchunk;
@@ -897,7 +898,7 @@ void launch_mg_smooth3(const Geo& g, const Geo& gw, int src, bool rz, const doub
                        double* xo, PcgState* st, RedBuf rb, hipStream_t s)
 {
     const Tile3Grid tg = mg_tile_grid(g);
-    const dim3 grid((tg.work + 7) / 8 * 8), blk(kBlock);
+    const dim3 grid((tg.work + 7) / 8 * 8), blk(kTileBlock);
     const Tile3Parent par{xc, pm.base, pm.sy, pm.sz, alpha};
 #define CF_MG_SMOOTH3(M, RZ)                                                                \
     hipLaunchKernelGGL((k_mg_smooth3<double, M, RZ>), grid, blk, 0, s, g, gw, tg, xi, b, par, \
@@ -932,7 +933,7 @@ static void mg_smooth3_f32(const Geo& g, const Geo& gw, int src, bool rz, const 
                            const float* b32, const double* r64, const Tile3ParentT<TP>& par,
                            float* xo, PcgState* st, RedBuf rb, hipStream_t s)
 {
-    const dim3 blk(kBlock);
+    const dim3 blk(kTileBlock);
     if (rz) {
         const Tile3Grid tg = mg_tile_grid<1>(g);
         const dim3 grid((tg.work + 7) / 8 * 8);
@@ -990,7 +991,7 @@ void launch_mg_resrestrict3(const Geo& g, const double* x, const double* b, doub
                             MgParentMap pm, const PcgState* st, hipStream_t s)
 {
     const Tile3Grid tg = mg_tile_grid(g);
-    hipLaunchKernelGGL(k_mg_resrestrict3<double>, dim3((tg.work + 7) / 8 * 8), dim3(kBlock), 0,
+    hipLaunchKernelGGL(k_mg_resrestrict3<double>, dim3((tg.work + 7) / 8 * 8), dim3(kTileBlock), 0,
                        s, g, tg, pm, x, b, bc, st);
 }
 
@@ -1000,7 +1001,7 @@ static void mg_resrestrict3_f32(const Geo& g, const float* x, const float* b, TO
 {
     const Tile3Grid tg = mg_tile_grid<CPL>(g);
     hipLaunchKernelGGL((k_mg_resrestrict3<float, CPL, PX, TO>), dim3((tg.work + 7) / 8 * 8),
-                       dim3(kBlock), 0, s, g, tg, pm, x, b, bc, st);
+                       dim3(kTileBlock), 0, s, g, tg, pm, x, b, bc, st);
 }
 
 void launch_mg_resrestrict3_f32(const Geo& g, const float* x, const float* b, double* bc,

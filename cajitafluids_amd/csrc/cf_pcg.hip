@@ -891,7 +891,7 @@ struct UpdEpiT
 // TR: the element type of r (float: an FP32 multigrid V-cycle's z, widened
 // on load; its values are the FP64 z's exactly).
 template <class T, int CPL, bool PX, class TR = T>
-__global__ __launch_bounds__(256) void k_pcg_dir_spmv3(
+__global__ __launch_bounds__(kTileBlock) void k_pcg_dir_spmv3(
     Geo g, Tile3Grid tg, const TR* __restrict__ r, const T* __restrict__ p_old,
     T* __restrict__ p_new, PcgState* st, RedBuf rb)
 {
@@ -904,12 +904,12 @@ __global__ __launch_bounds__(256) void k_pcg_dir_spmv3(
     DirEpiT<T, CPL> epi{g, p_new};
     tile3_march<T, true, CPL, PX, false, TR>(g, tg, r, p_old, first, beta, pl, epi);
     dd_norm(epi.hi[0], epi.lo[0]);
-    grid_reduce<1>(epi.hi, epi.lo, rb,
+    grid_reduce<1, kTileBlock>(epi.hi, epi.lo, rb,
                    [&](double* sm) { st->alpha = st->rho / sm[0]; });
 }
 
 template <class T, int CPL, bool PX, int XM, bool R32>
-__global__ __launch_bounds__(256) void k_pcg_update3(
+__global__ __launch_bounds__(kTileBlock) void k_pcg_update3(
     Geo g, Tile3Grid tg, T* __restrict__ x, T* __restrict__ r,
     const T* __restrict__ p, const T* __restrict__ p_lag, float* __restrict__ r32,
     PcgState* st, RedBuf rb)
@@ -925,7 +925,7 @@ __global__ __launch_bounds__(256) void k_pcg_update3(
     tile3_march<T, false, CPL, PX, true>(g, tg, p, nullptr, false, (T)0, pl, epi);
     dd_norm(epi.hi[0], epi.lo[0]);
     dd_norm(epi.hi[1], epi.lo[1]);
-    grid_reduce<2>(epi.hi, epi.lo, rb, [&](double* sm) {
+    grid_reduce<2, kTileBlock>(epi.hi, epi.lo, rb, [&](double* sm) {
         if (XM == X_SKIP) // every block has read the state by now
             st->alpha_lag = st->alpha;
         finish_update(st, sm[0], sm[1]);
@@ -1160,7 +1160,7 @@ static void dir_spmv3_t(const Geo& g, const T* r, const T* p_old, T* p_new, PcgS
                         RedBuf rb, hipStream_t s)
 {
     hipLaunchKernelGGL((k_pcg_dir_spmv3<T, cpl_of<T>(), PX>),
-                       dim3(tile3_blocks<cpl_of<T>()>(g)), dim3(kBlock), 0, s, g,
+                       dim3(tile3_blocks<cpl_of<T>()>(g)), dim3(kTileBlock), 0, s, g,
                        tile3_grid<cpl_of<T>()>(g), r, p_old, p_new, st, rb);
 }
 
@@ -1169,7 +1169,7 @@ static void update3_t(const Geo& g, T* x, T* r, const T* p, const T* p_lag, floa
                       PcgState* st, RedBuf rb, hipStream_t s)
 {
     hipLaunchKernelGGL((k_pcg_update3<T, cpl_of<T>(), PX, XM, R32>),
-                       dim3(tile3_blocks<cpl_of<T>()>(g)), dim3(kBlock), 0, s, g,
+                       dim3(tile3_blocks<cpl_of<T>()>(g)), dim3(kTileBlock), 0, s, g,
                        tile3_grid<cpl_of<T>()>(g), x, r, p, p_lag, r32, st, rb);
 }
 
@@ -1200,7 +1200,7 @@ void launch_pcg_dir_spmv_z32(const Geo& g, const float* z, const double* p_old, 
     if (g.dim != 3)
         throw CfError(CF_EINVAL, "an FP32 z is a 3-D multigrid V-cycle's");
     hipLaunchKernelGGL((k_pcg_dir_spmv3<double, 1, false, float>), dim3(tile3_blocks<1>(g)),
-                       dim3(kBlock), 0, s, g, tile3_grid<1>(g), z, p_old, p_new, st, rb);
+                       dim3(kTileBlock), 0, s, g, tile3_grid<1>(g), z, p_old, p_new, st, rb);
 }
 
 template <class T>

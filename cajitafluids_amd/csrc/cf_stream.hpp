@@ -34,10 +34,10 @@ namespace cfmi
 // read memory, not its own XCD's L2), reduces them and runs `fin` on thread
 // 0.  No fence is issued: an agent-scope fence would write back and
 // invalidate the XCD's whole L2 on every block.
-template <int NV>
+template <int NV, int BS = kBlock>
 __device__ __forceinline__ void block_reduce(double (&hi)[NV], double (&lo)[NV])
 {
-    __shared__ double sh[NV][2][kBlock / kWave];
+    __shared__ double sh[NV][2][BS / kWave];
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) {
 #pragma unroll
@@ -62,18 +62,18 @@ __device__ __forceinline__ void block_reduce(double (&hi)[NV], double (&lo)[NV])
         for (int v = 0; v < NV; ++v) {
             hi[v] = sh[v][0][0];
             lo[v] = sh[v][1][0];
-            for (int w = 1; w < kBlock / kWave; ++w)
+            for (int w = 1; w < BS / kWave; ++w)
                 dd_add(hi[v], lo[v], sh[v][0][w], sh[v][1][w]);
         }
     }
     __syncthreads();
 }
 
-template <int NV, class Fin>
+template <int NV, int BS = kBlock, class Fin>
 __device__ __forceinline__ void grid_reduce(double (&hi)[NV], double (&lo)[NV],
                                             RedBuf rb, Fin&& fin)
 {
-    block_reduce<NV>(hi, lo);
+    block_reduce<NV, BS>(hi, lo);
     __shared__ int last;
     if (threadIdx.x == 0) {
         double* part = rb.partials + (size_t)blockIdx.x * NV * 2;
@@ -122,7 +122,7 @@ __device__ __forceinline__ void grid_reduce(double (&hi)[NV], double (&lo)[NV],
         h2[v] = 0.0;
         l2[v] = 0.0;
     }
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock) {
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) {
         double* part = rb.partials + (size_t)b * NV * 2;
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
@@ -133,7 +133,7 @@ __device__ __forceinline__ void grid_reduce(double (&hi)[NV], double (&lo)[NV],
             dd_add(h2[v], l2[v], ph, pl);
         }
     }
-    block_reduce<NV>(h2, l2);
+    block_reduce<NV, BS>(h2, l2);
     if (threadIdx.x == 0) {
         double sum[NV];
 #pragma unroll

@@ -39,17 +39,26 @@
 
 namespace cfmi
 {
+// Lanes per block of the tile kernels (4 waves: 16-row tiles).  With 512
+// (8 waves, 32-row tiles: half the y-halo rows per cell) the FP64 step was
+// the same (284.9 against 285.1 ms at 256^3) and the FP32-refined one 1%
+// slower: the halo rows' reads hit the caches.
+#ifndef CF_TILE_BLOCK
+#define CF_TILE_BLOCK 256
+#endif
+constexpr int kTileBlock = CF_TILE_BLOCK;
+
 namespace tile3
 {
-constexpr int LX = 64;          // lanes along x
-constexpr int NW = kBlock / LX; // waves per block
+constexpr int LX = 64;              // lanes along x
+constexpr int NW = kTileBlock / LX; // waves per block
 constexpr int RY = 4;           // rows per lane
 constexpr int TY = NW * RY;     // tile rows
 constexpr int NS = RY + 1;      // load slots per lane: RY own rows + 1 halo
 // halo slots: the two y-halo rows (one lane per CPL cells) and the two
 // x-halo columns (one lane per cell)
 constexpr int NHALO = 2 * LX + 2 * TY;
-static_assert(NHALO <= kBlock, "one halo slot per lane at most");
+static_assert(NHALO <= kTileBlock, "one halo slot per lane at most");
 template <int CPL>
 constexpr int tx()
 {
