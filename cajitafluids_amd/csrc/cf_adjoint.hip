@@ -210,12 +210,15 @@ struct DirectScatter
 };
 
 // The LDS windows of k_adj_advect_win: per array, the raw local indices
-// [tile origin - M, tile end + M) of a 32 x 4 x 2 tile of lattice points.
+// [slab origin - M, slab end + M) of a 32 x 4 x 2 slab of lattice points.
+// The WZ planes form a ring along z: plane z sits in slot (z - zbase) mod WZ.
 namespace adjwin
 {
 constexpr int TX = 32, TY = 4, TZ = 2, M = 3;
 constexpr int WX = TX + 2 * M, WY = TY + 2 * M, WZ = TZ + 2 * M;
-constexpr int WN = WX * WY * WZ; // 3040 doubles per array, 97 KB for four
+constexpr int WP = WX * WY;      // cells of one window plane
+constexpr int WN = WP * WZ;      // 3040 doubles per array, 97 KB for four
+static_assert((WZ & (WZ - 1)) == 0, "the z ring indexes with a mask");
 // the Makefile's ADJ_WIN_LDS: check_lds.py fails the build unless the
 // compiled kernels hold exactly this much LDS
 static_assert(4 * WN * sizeof(double) == 97280, "LDS windows of k_adj_advect_win");
@@ -226,7 +229,8 @@ struct WindowedScatter
     const Geo& g;
     const Fields& bar;
     double (*win)[adjwin::WN];
-    int wlo[3];
+    int wlo[3]; // the window's lowest raw local index per axis
+    int zbase;  // the plane in ring slot 0 (z - zbase >= 0 for every window plane)
     template <int DIM, int ORDER>
     __device__ __forceinline__ void add(int e, const Taps<DIM, ORDER>& t, double val) const
     {
@@ -238,16 +242,18 @@ struct WindowedScatter
             scatter<DIM, ORDER>(g, e, bar.e[e], t, val);
             return;
         }
-        double* w = win[e] + (t.s0[0] - wlo[0]) + WX * ((t.s0[1] - wlo[1]) + WY * (t.s0[2] - wlo[2]));
+        double* w = win[e] + (t.s0[0] - wlo[0]) + WX * (t.s0[1] - wlo[1]);
+        const int zs = t.s0[2] - zbase;
 #pragma unroll 1
         for (int c = 0; c < ORDER + 1; ++c) {
             const double wz = pick(t.w[2], c) * val;
+            double* wp = w + WP * ((zs + c) & (WZ - 1));
 #pragma unroll
             for (int b = 0; b < ORDER + 1; ++b) {
                 const double wyz = t.w[1][b] * wz;
 #pragma unroll
                 for (int a = 0; a < ORDER + 1; ++a)
-                    atomicAdd(w + a + WX * (b + WY * c), t.w[0][a] * wyz);
+                    atomicAdd(wp + a + WX * b, t.w[0][a] * wyz);
             }
         }
     }
@@ -369,15 +375,19 @@ __global__ __launch_bounds__(256) void k_adj_advect(Geo g, Fields cur, Fields ob
     }
 }
 
-// 3-D: a block's 32 x 4 x 2 lattice points accumulate their scatters in LDS
-// windows (one per array, the tile plus M cells a side), then add each
-// non-zero window cell to its owned index once.  A point's traces stay
-// within about one cell at the clamped time step, so nearly every sample
-// lands in the window; the global atomics drop from ~136 per point to the
-// window's cells per block.
+// 3-D: a block marches a column of 32 x 4 lattice points through `slabs`
+// slabs of 2 planes.  Each slab's points accumulate their scatters in LDS
+// windows (one per array, the slab plus M cells a side); the windows' planes
+// form a ring along z, so when the march moves on by a slab, the TZ planes
+// below the new window are complete: each non-zero cell of them is added to
+// its owned index once and the slots are cleared for the planes entering at
+// the top.  A point's traces stay within about one cell at the clamped time
+// step, so nearly every sample lands in the window; the global atomics drop
+// from ~136 per point to the window cells a slab leaves behind (2 planes of
+// 38 x 10 per 256 points, plus the last window of the column).
 template <int FO, int VO>
 __global__ __launch_bounds__(256) void k_adj_advect_win(Geo g, Fields cur, Fields obar,
-                                                        Fields bar)
+                                                        Fields bar, int slabs)
 {
     using namespace adjwin;
     __shared__ double win[4][WN];
@@ -385,40 +395,55 @@ __global__ __launch_bounds__(256) void k_adj_advect_win(Geo g, Fields cur, Field
     for (int q = t; q < 4 * WN; q += TX * TY * TZ)
         win[q / WN][q % WN] = 0.0;
     const int i0 = g.h + blockIdx.x * TX, j0 = g.h + blockIdx.y * TY;
-    const int k0 = g.own_lo(2) + blockIdx.z * TZ;
-    const WindowedScatter sc{g, bar, win, {i0 - M, j0 - M, k0 - M}};
+    const int nslab = (g.uni_hi[2] - g.own_lo(2) + TZ - 1) / TZ;
+    const int kb = g.own_lo(2) + (int)blockIdx.z * slabs * TZ;
+    const int kend = g.own_lo(2) + min(nslab, ((int)blockIdx.z + 1) * slabs) * TZ;
+    const int zbase = kb - M;
     __syncthreads();
-    const int l[3] = {i0 + (int)threadIdx.x, j0 + (int)threadIdx.y, k0 + (int)threadIdx.z};
-    const int64_t c = g.idx(l[0], l[1], l[2]);
 #pragma unroll 1
-    for (int e = 0; e <= 3; ++e) {
-        const bool own = l[0] < g.own_hi[e][0] && l[1] < g.own_hi[e][1] && l[2] < g.own_hi[e][2];
-        if (!own)
-            continue;
-        const double ob = obar.e[e][c];
-        if (ob == 0.0)
-            continue;
-        double x0[3];
+    for (int k0 = kb; k0 < kend; k0 += TZ) {
+        const WindowedScatter sc{g, bar, win, {i0 - M, j0 - M, k0 - M}, zbase};
+        const int l[3] = {i0 + (int)threadIdx.x, j0 + (int)threadIdx.y, k0 + (int)threadIdx.z};
+        const int64_t c = g.idx(l[0], l[1], l[2]);
+#pragma unroll 1
+        for (int e = 0; e <= 3; ++e) {
+            const bool own = l[0] < g.own_hi[e][0] && l[1] < g.own_hi[e][1] &&
+                             l[2] < g.own_hi[e][2];
+            if (!own)
+                continue;
+            const double ob = obar.e[e][c];
+            if (ob == 0.0)
+                continue;
+            double x0[3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d)
-            x0[d] = coord(g, e, d, l[d]);
-        advect_point_adj<3, FO, VO>(g, e, cur, x0, ob, sc);
-    }
-    __syncthreads();
-    // flush: window cell -> its owned index (wrapped; wall ghosts dropped)
-#pragma unroll 1
-    for (int a = 0; a < 4; ++a) {
-        for (int q = t; q < WN; q += TX * TY * TZ) {
-            const double v = win[a][q];
-            if (v == 0.0)
-                continue;
-            const int ox = owned_index(g, a, 0, sc.wlo[0] + q % WX);
-            const int oy = owned_index(g, a, 1, sc.wlo[1] + (q / WX) % WY);
-            const int oz = owned_index(g, a, 2, sc.wlo[2] + q / (WX * WY));
-            if (ox < 0 || oy < 0 || oz < 0)
-                continue;
-            atomicAdd(bar.e[a] + g.idx(ox, oy, oz), v);
+            for (int d = 0; d < 3; ++d)
+                x0[d] = coord(g, e, d, l[d]);
+            advect_point_adj<3, FO, VO>(g, e, cur, x0, ob, sc);
         }
+        __syncthreads();
+        // flush the planes [k0 - M, k0 - M + np): the TZ lowest, which no
+        // later slab's window holds, or after the last slab the whole window;
+        // each cell goes to its owned index (wrapped; wall ghosts dropped)
+        // and its slot is cleared
+        const int np = k0 + TZ >= kend ? WZ : TZ;
+#pragma unroll 1
+        for (int a = 0; a < 4; ++a) {
+            for (int q = t; q < np * WP; q += TX * TY * TZ) {
+                const int z = k0 - M + q / WP, r = q % WP;
+                double* cell = &win[a][WP * ((z - zbase) & (WZ - 1)) + r];
+                const double v = *cell;
+                if (v == 0.0)
+                    continue;
+                *cell = 0.0;
+                const int ox = owned_index(g, a, 0, sc.wlo[0] + r % WX);
+                const int oy = owned_index(g, a, 1, sc.wlo[1] + r / WX);
+                const int oz = owned_index(g, a, 2, z);
+                if (ox < 0 || oy < 0 || oz < 0)
+                    continue;
+                atomicAdd(bar.e[a] + g.idx(ox, oy, oz), v);
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -436,9 +461,17 @@ static void adj_advect_t(const Geo& g, Fields cur, Fields obar, Fields bar,
     if constexpr (DIM == 3) {
         using namespace adjwin;
         const dim3 blk(TX, TY, TZ);
-        const int ex = g.uni_hi[0] - g.h, ey = g.uni_hi[1] - g.h, ez = g.uni_hi[2] - g.h;
-        const dim3 grid((ex + TX - 1) / TX, (ey + TY - 1) / TY, (ez + TZ - 1) / TZ);
-        hipLaunchKernelGGL((k_adj_advect_win<FO, VO>), grid, blk, 0, s, g, cur, obar, bar);
+        const int ex = g.uni_hi[0] - g.h, ey = g.uni_hi[1] - g.h;
+        const int nslab = (g.uni_hi[2] - g.own_lo(2) + TZ - 1) / TZ;
+        const int cols = ((ex + TX - 1) / TX) * ((ey + TY - 1) / TY);
+        // columns of up to 16 slabs, shorter while that leaves fewer than
+        // 2048 blocks (8 per CU)
+        int slabs = 16;
+        while (slabs > 1 && cols * ((nslab + slabs - 1) / slabs) < 2048)
+            slabs /= 2;
+        const dim3 grid((ex + TX - 1) / TX, (ey + TY - 1) / TY, (nslab + slabs - 1) / slabs);
+        hipLaunchKernelGGL((k_adj_advect_win<FO, VO>), grid, blk, 0, s, g, cur, obar, bar,
+                           slabs);
     } else {
         dim3 blk(64, 4, 1);
         hipLaunchKernelGGL((k_adj_advect<DIM, FO, VO>), union_grid(g, blk), blk, 0, s,
