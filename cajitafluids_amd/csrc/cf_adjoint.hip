@@ -83,8 +83,11 @@ __device__ __forceinline__ T pick(const T (&a)[N], int i)
     return r;
 }
 
-// d/dp of the sample of f.
-template <int DIM, int ORDER>
+// d/dp of the sample of f.  3-D: BATCH loads the taps of all z planes
+// before they are used (in flight together); otherwise one plane at a time,
+// which the kernels with cubic velocity samples need to stay within the
+// register file.
+template <int DIM, int ORDER, bool BATCH = true>
 __device__ __forceinline__ void sample_grad(const Geo& g, const Taps<DIM, ORDER>& t,
                                             const double* __restrict__ f,
                                             double (&gr)[3])
@@ -104,9 +107,8 @@ __device__ __forceinline__ void sample_grad(const Geo& g, const Taps<DIM, ORDER>
             gr[0] = fma(t.w[1][b], sx, gr[0]);
             gr[1] = fma(t.dw[1][b], sv, gr[1]);
         }
-    } else {
+    } else if constexpr (!BATCH) {
         const int64_t pz = g.px * g.py;
-        // z planes one at a time (16 loads in flight instead of 64)
 #pragma unroll 1
         for (int c = 0; c < ORDER + 1; ++c) {
             double gx = 0.0, gy = 0.0, vv = 0.0;
@@ -128,6 +130,37 @@ __device__ __forceinline__ void sample_grad(const Geo& g, const Taps<DIM, ORDER>
             gr[0] = fma(wz, gx, gr[0]);
             gr[1] = fma(wz, gy, gr[1]);
             gr[2] = fma(pick(t.dw[2], c), vv, gr[2]);
+        }
+    } else {
+        const int64_t pz = g.px * g.py;
+        const double* base = f + g.idx(t.s0[0], t.s0[1], t.s0[2]);
+        // each z plane's taps are loaded before any is used, so that they
+        // are in flight together (the planes are unrolled: constant indices
+        // keep the weights in registers)
+#pragma unroll
+        for (int c = 0; c < ORDER + 1; ++c) {
+            double fv[ORDER + 1][ORDER + 1];
+#pragma unroll
+            for (int b = 0; b < ORDER + 1; ++b)
+#pragma unroll
+                for (int a = 0; a < ORDER + 1; ++a)
+                    fv[b][a] = base[c * pz + b * g.px + a];
+            double gx = 0.0, gy = 0.0, vv = 0.0;
+#pragma unroll
+            for (int b = 0; b < ORDER + 1; ++b) {
+                double sx = 0.0, sv = 0.0;
+#pragma unroll
+                for (int a = 0; a < ORDER + 1; ++a) {
+                    sx = fma(t.dw[0][a], fv[b][a], sx);
+                    sv = fma(t.w[0][a], fv[b][a], sv);
+                }
+                gx = fma(t.w[1][b], sx, gx);
+                gy = fma(t.dw[1][b], sv, gy);
+                vv = fma(t.w[1][b], sv, vv);
+            }
+            gr[0] = fma(t.w[2][c], gx, gr[0]);
+            gr[1] = fma(t.w[2][c], gy, gr[1]);
+            gr[2] = fma(t.dw[2][c], vv, gr[2]);
         }
     }
 }
@@ -191,7 +224,7 @@ __device__ __forceinline__ void velocity_jac(const Geo& g, const Fields& cur,
     for (int d = 0; d < DIM; ++d) {
         Taps<DIM, VO> t;
         make_taps<DIM, VO>(g, d + 1, x, t);
-        sample_grad<DIM, VO>(g, t, cur.e[d + 1], J[d]);
+        sample_grad<DIM, VO, VO == 1>(g, t, cur.e[d + 1], J[d]);
     }
 }
 
@@ -244,9 +277,9 @@ struct WindowedScatter
         }
         double* w = win[e] + (t.s0[0] - wlo[0]) + WX * (t.s0[1] - wlo[1]);
         const int zs = t.s0[2] - zbase;
-#pragma unroll 1
+#pragma unroll
         for (int c = 0; c < ORDER + 1; ++c) {
-            const double wz = pick(t.w[2], c) * val;
+            const double wz = t.w[2][c] * val;
             double* wp = w + WP * ((zs + c) & (WZ - 1));
 #pragma unroll
             for (int b = 0; b < ORDER + 1; ++b) {
@@ -303,7 +336,7 @@ __device__ __forceinline__ void advect_point_adj(const Geo& g, int e, const Fiel
         Taps<DIM, FO> tf;
         make_taps<DIM, FO>(g, e, tr, tf);
         sc.template add<DIM, FO>(e, tf, ob);
-        sample_grad<DIM, FO>(g, tf, cur.e[e], gf);
+        sample_grad<DIM, FO, VO == 1>(g, tf, cur.e[e], gf);
     }
     // back through the RK3 combination (x0 is fixed: no Jacobian there)
     double v0b[3] = {0, 0, 0}, v1b[3] = {0, 0, 0}, v2b[3] = {0, 0, 0};
