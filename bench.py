@@ -539,9 +539,49 @@ def adjoint_leg(cfg, nsteps, ckpt, init, loss_grad):
             "forward_s": round(t_fwd, 3), "adjoint_s": round(t_adj, 3),
             "adjoint_over_forward": round(t_adj / t_fwd, 2),
             "grad_u_norm": float(np.linalg.norm(g["du"])),
+            "dvel": [float(v) for v in g["dvel"]],
             "note": "adjoint_s includes the host copies of dL/dq_T in and of the "
-                    "gradients out (cf_adjoint_run takes host arrays)",
+                    "gradients out (cf_adjoint_run takes host arrays); adjoint_device_s, "
+                    "where present, is cf_adjoint_run_device on device arrays",
             "_du": g["du"]}
+
+
+def adjoint_device_s(cfg, nsteps, ckpt, init):
+    """Wall time of cf_adjoint_run_device with every array in device memory
+    (no host copies), for the loss 0.5 ||q_T||^2: dL/dq_T = q_T, copied
+    device to device from a forward run's final state.  Returns the time and
+    dL/d(inflow velocity)."""
+    import ctypes as C
+    from cajitafluids_amd import _lib
+    from cajitafluids_amd.solver import Solver, padded_layout
+    _lib.load()
+    hip = C.CDLL("libamdhip64.so.7")  # the runtime libcfmi.so links
+    nbytes = padded_layout(cfg)[3] * 8
+    bufs = []
+    try:
+        for _ in range(5):
+            p = C.c_void_p()
+            if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
+                raise RuntimeError("hipMalloc failed")
+            bufs.append(p.value)
+        with Solver(cfg) as s:
+            init(s)
+            s.setup()
+            for _ in range(nsteps):
+                s.step()
+            s.copy_out_device(0, bufs[0])
+            s.sync()
+        with Solver(cfg) as s:
+            init(s)
+            s.sync()
+            t0 = time.perf_counter()
+            dvel = s.adjoint_run_device(nsteps, ckpt, *bufs)
+            t = time.perf_counter() - t0
+        print(f"device adjoint: {t:.3f} s", file=sys.stderr, flush=True)
+        return t, dvel
+    finally:
+        for p in bufs:
+            hip.hipFree(C.c_void_p(p))
 
 
 def with_mg(cfg):
@@ -604,6 +644,8 @@ def adjoint_legs():
                                           lambda s: s.get(0))
     out["periodic_box_384_mg"] = adjoint_leg(with_mg(c5), K.PERIODIC_BOX_STEPS, K.PERIODIC_BOX_CKPT,
                                              init5, lambda s: s.get(0))
+    t, _ = adjoint_device_s(with_mg(c5), K.PERIODIC_BOX_STEPS, K.PERIODIC_BOX_CKPT, init5)
+    out["periodic_box_384_mg"]["adjoint_device_s"] = round(t, 3)
     # the workload's FP32-refined precision with the multigrid solver: the
     # V-cycle's finest level in FP32
     c5m = with_mg(c5)
