@@ -235,7 +235,7 @@ __global__ __launch_bounds__(256) void k_pcg_update(
 // chain (about 7.5 us each at 128^2), not to bandwidth.  Here one
 // cooperative launch of G workgroups runs all iterations:
 //   - workgroup g owns the rows [g R, g R + R) as a strip; lane t of it owns
-//     strip cells t + 256 m (m < kCoopCpt, x fastest); r, x and q = A p of
+//     strip cells t + 256 m (m < CPT, 4 or 8, x fastest); r, x and q = A p of
 //     those cells stay in registers;
 //   - the direction lives in the strip's LDS window: its rows, one halo row
 //     above and below, and a one-cell ring in x.  Ring cells of periodic
@@ -278,15 +278,28 @@ struct CoopGrid
     int G, R, e0, nwin;
     int c00; // padded index of owned cell (0, 0)
     int px;
+    int cpt; // cells per lane the kernel is built for: 4 or kCoopCpt
 };
 
-// Strips of as many whole rows as kCoopCells holds: the fewest workgroups.
-// (Shorter strips, at two cells per lane, took 3% longer at 128^2: the
-// waits for the other strips, not the strip's own work, set the pace.)
+// Strips of whole rows, at most cpt cells per lane.  When strips of 4 cells
+// per lane leave at most 16 workgroups (64 wave partials, one per lane of
+// the gather), those: 8.1 against 9.0 us per iteration at 128^2, where the
+// strip's own work is a good part of an iteration.  Otherwise as many rows
+// as kCoopCells holds, the fewest workgroups: more strips cost more in the
+// waits than they save in work (at 512^2, 256 strips of 2 rows took 14.1
+// us per iteration against 13.05 for 128 strips of 4; strips of 2 cells per
+// lane took 3% longer at 128^2).
 inline CoopGrid coop_grid(const Geo& g)
 {
     CoopGrid cg;
-    cg.R = kCoopCells / g.n[0];
+    const int r4 = (kCoopThreads * 4) / g.n[0];
+    if (r4 > 0 && ((g.n[1] + r4 - 1) / r4) * kCoopWaves <= kWave) {
+        cg.cpt = 4;
+        cg.R = r4;
+    } else {
+        cg.cpt = kCoopCpt;
+        cg.R = kCoopCells / g.n[0];
+    }
     if (cg.R > g.n[1])
         cg.R = g.n[1];
     cg.G = (g.n[1] + cg.R - 1) / cg.R;
@@ -476,7 +489,7 @@ constexpr int kCoopOwned = 1 << 20;
 constexpr int kCoopTop = 1 << 21;
 constexpr int kCoopBottom = 1 << 22;
 
-template <int PPL, bool WP>
+template <int PPL, bool WP, int CPT>
 __global__ __launch_bounds__(kCoopThreads) void k_pcg_solve_coop(
     Geo g, CoopGrid cg, double* __restrict__ x, double* __restrict__ r,
     double* __restrict__ pg, PcgState* st, CoopXch xc, int max_its)
@@ -546,10 +559,10 @@ __global__ __launch_bounds__(kCoopThreads) void k_pcg_solve_coop(
                 hr[e] = r[cg.c00 + i + (int64_t)cg.px * (side == 0 ? jtop : jbot)];
         }
 
-    int info[kCoopCpt];
-    double rv[kCoopCpt], qv[kCoopCpt], xv[kCoopCpt];
+    int info[CPT];
+    double rv[CPT], qv[CPT], xv[CPT];
 #pragma unroll
-    for (int m = 0; m < kCoopCpt; ++m) {
+    for (int m = 0; m < CPT; ++m) {
         const int L = t + kCoopThreads * m;
         const int i = L % n0, jj = L / n0;
         info[m] = 1 + e0; // window index of strip cell (0, 0)
@@ -578,15 +591,15 @@ __global__ __launch_bounds__(kCoopThreads) void k_pcg_solve_coop(
         // cells (staged in qv, free until the stencil) and its halo rows
         if (first) {
 #pragma unroll
-            for (int m = 0; m < kCoopCpt; ++m)
+            for (int m = 0; m < CPT; ++m)
                 qv[m] = tab[7 + ((info[m] >> 16) & 7)] * rv[m];
         } else {
 #pragma unroll
-            for (int m = 0; m < kCoopCpt; ++m)
+            for (int m = 0; m < CPT; ++m)
                 qv[m] = fma(beta, pw[info[m] & 0xffff], tab[7 + ((info[m] >> 16) & 7)] * rv[m]);
         }
 #pragma unroll
-        for (int m = 0; m < kCoopCpt; ++m)
+        for (int m = 0; m < CPT; ++m)
             if (info[m] & kCoopOwned)
                 pw[info[m] & 0xffff] = qv[m];
         for (int e = t; e < 2 * n0; e += kCoopThreads) {
@@ -609,7 +622,7 @@ __global__ __launch_bounds__(kCoopThreads) void k_pcg_solve_coop(
         // q = A p in the stencil order {0}, -x, +x, -y, +y; p.q
         double h1[1] = {0.0}, l1[1] = {0.0};
 #pragma unroll
-        for (int m = 0; m < kCoopCpt; ++m) {
+        for (int m = 0; m < CPT; ++m) {
             const int im = info[m];
             const double* pp = pw + (im & 0xffff);
             const double pc = pp[0];
@@ -642,7 +655,7 @@ __global__ __launch_bounds__(kCoopThreads) void k_pcg_solve_coop(
         // unchanged.
         double hi[2] = {0.0, 0.0}, lo[2] = {0.0, 0.0};
 #pragma unroll
-        for (int m = 0; m < kCoopCpt; ++m) {
+        for (int m = 0; m < CPT; ++m) {
             const int im = info[m];
             xv[m] = fma(alpha, pw[im & 0xffff], xv[m]);
             const double rn = fma(-alpha, qv[m], rv[m]);
@@ -703,7 +716,7 @@ __global__ __launch_bounds__(kCoopThreads) void k_pcg_solve_coop(
     if (gid == 0 && t == 0)
         *st = ls;
 #pragma unroll
-    for (int m = 0; m < kCoopCpt; ++m)
+    for (int m = 0; m < CPT; ++m)
         if (info[m] & kCoopOwned) {
             const int li = info[m] & 0xffff;
             const int64_t c = padded(li);
@@ -1304,12 +1317,13 @@ bool launch_pcg_solve_coop(const Geo& g, double* x, double* r, double* p, PcgSta
     // per lane (up to 64 strips: one store-acknowledgement wait per
     // reduction); beyond that each workgroup publishes one partial, summed
     // over its waves (a second wait, but 4 times fewer reads).  Measured per
-    // 2-D iteration: 8.6 us at 128^2 (wave partials) against 9.1, 12.8 us at
-    // 512^2 (workgroup partials) against 13.9.
+    // 2-D iteration: 8.6 us at 128^2 (wave partials, 8 cells per lane)
+    // against 9.1, 12.8 us at 512^2 (workgroup partials) against 13.9.
     const int nwp = cg.G * kCoopWaves;
-    const void* fn = nwp <= kWave       ? (const void*)k_pcg_solve_coop<1, true>
-                     : nwp <= 4 * kWave ? (const void*)k_pcg_solve_coop<4, true>
-                                        : (const void*)k_pcg_solve_coop<4, false>;
+    const void* fn = cg.cpt == 4        ? (const void*)k_pcg_solve_coop<1, true, 4>
+                     : nwp <= kWave     ? (const void*)k_pcg_solve_coop<1, true, kCoopCpt>
+                     : nwp <= 4 * kWave ? (const void*)k_pcg_solve_coop<4, true, kCoopCpt>
+                                        : (const void*)k_pcg_solve_coop<4, false, kCoopCpt>;
     static_assert(kCoopMaxGroups <= 4 * kWave, "gather bucket");
     const hipError_t e = hipLaunchCooperativeKernel(fn, dim3(cg.G), dim3(kCoopThreads), args,
                                                     (unsigned)lds, s);

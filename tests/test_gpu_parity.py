@@ -132,7 +132,7 @@ COOP_CASES = {
     "periodic_x_300x200": _cfg2(300, 200, periodic=(1, 0)),
     "periodic_y_100x90": _cfg2(100, 90, periodic=(0, 1)),
     "periodic_xy_64x48": _cfg2(64, 48, periodic=(1, 1)),
-    "periodic_y_one_strip_40x30": _cfg2(40, 30, periodic=(0, 1)),
+    "periodic_y_one_strip_40x20": _cfg2(40, 20, periodic=(0, 1)),
     "free_walls_77x61": _cfg2(77, 61, walls=[[K.FREE, K.SOLID], [K.SOLID, K.FREE],
                                              [K.SOLID, K.SOLID]]),
     "wide_rows_1000x20": _cfg2(1000, 20),
@@ -178,11 +178,13 @@ def test_pcg_2d_cooperative_strips(cf, oracle_mod, name):
 
 @pytest.mark.parametrize("name,maxit", [("periodic_x_300x200", 1), ("periodic_x_300x200", 2),
                                         ("periodic_x_300x200", 5), ("periodic_x_300x200", 300),
-                                        ("periodic_xy_64x48", 1), ("periodic_xy_64x48", 6)])
+                                        ("periodic_xy_64x48", 1), ("periodic_xy_64x48", 6),
+                                        ("solid_128", 300)])
 def test_pcg_2d_cooperative_truncated(cf, oracle_mod, name, maxit):
     """Solves stopped by the iteration cap, inside the first launch and (300)
     in a second one that resumes from the stored state (the first launch of
-    a handle's first solve runs 256 iterations)."""
+    a handle's first solve runs 256 iterations), with strips of 8 cells per
+    lane (300x200) and of 4 (64x48, 128^2)."""
     print(f"cooperative pcg truncated: {name} maxit {maxit}", flush=True)
     cfg = dict(COOP_CASES[name], cg_max_iter=maxit, cg_tol=1e-30)
     st = _coop_solve(cf, oracle_mod, cfg, 9)
