@@ -287,8 +287,9 @@ struct CoopGrid
 // strip's own work is a good part of an iteration.  Otherwise as many rows
 // as kCoopCells holds, the fewest workgroups: more strips cost more in the
 // waits than they save in work (at 512^2, 256 strips of 2 rows took 14.1
-// us per iteration against 13.05 for 128 strips of 4; strips of 2 cells per
-// lane took 3% longer at 128^2).
+// us per iteration against 13.05 for 128 strips of 4, and 64 strips of 8
+// rows at 16 cells per lane 15.7; strips of 2 cells per lane took 3% longer
+// at 128^2).
 inline CoopGrid coop_grid(const Geo& g)
 {
     CoopGrid cg;

@@ -137,6 +137,9 @@ COOP_CASES = {
                                              [K.SOLID, K.SOLID]]),
     "wide_rows_1000x20": _cfg2(1000, 20),
     "unpreconditioned_96x80": _cfg2(96, 80, periodic=(1, 0), preconditioner=K.PRECOND_NONE),
+    # workgroup partials (75 and 138 strips)
+    "periodic_y_512x300": _cfg2(512, 300, periodic=(0, 1)),
+    "tall_256x1100": _cfg2(256, 1100),
 }
 
 
